@@ -1,0 +1,142 @@
+"""Benchmark: batched Goldilocks NTT on one MI355X.
+
+Workload (the plain batched-NTT configuration): 256 columns x 2^20 u64
+Goldilocks elements, column-major with stride 2^20, uniform draws, resident
+in HBM.  One step = four out-of-place calls over the whole matrix: forward and
+inverse, each with natural input (n2b) and bit-reversed input (b2n).
+`value` = transformed elements per second over the K timed steps.
+
+roofline: HBM bound.  Algorithmic bytes per transform call = 16 B x elements
+(each element read once and written once).  `achieved` uses the average
+duration of one call measured with HIP events on the stream the library
+launches on (torch's current stream) over the timed region.
+
+cpu_baseline: the oracle's multithreaded restatement of boojum's CPU NTT
+(oracle/gl_oracle.c), timed on a bounded sample of the same workload.
+
+Prints ONE JSON line.  `--gpus` above 1 is refused: this build measures one
+GPU's share (columns are independent; see DESIGN.md for sharding).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "era-boojum-cuda_amd"))
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "Goldilocks NTT/LDE elems/s & %HBM peak; ragged-batched vs plain; p50/p99 latency"
+
+
+def uniform_field(count, gen):
+    """Uniform draws in [0, p) as int64 bit patterns: full-range u64 draws,
+    the 2^32 - 1 values >= p folded down by p (probability 2^-32 each)."""
+    import torch
+    x = torch.randint(-2**63, 2**63 - 1, (count,), dtype=torch.int64, device="cuda", generator=gen)
+    big = (x < 0) & (x >= -(2**32 - 1))  # u64 value >= p
+    return torch.where(big, x + (2**32 - 1), x)
+
+
+def cpu_baseline(log_n, cols, threads):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    import oracle
+    n = 1 << log_n
+    inp = oracle.matrix_uniform(cols, n, 2)
+    out = np.empty_like(inp)
+    t0 = time.perf_counter()
+    oracle.batch_ntt(inp, log_n, cols, n, n, False, False, out=out, threads=threads)
+    dt = time.perf_counter() - t0
+    return {"value": cols * n / dt, "unit": "elem/s", "cores": threads, "kind": "port",
+            "sample": f"{cols} columns x 2^{log_n}, forward n2b, {dt:.2f} s wall"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--log-n", type=int, default=20)
+    ap.add_argument("--cols", type=int, default=256)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-cols", type=int, default=64)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (columns shard with no "
+              f"communication; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from boojum_cuda_amd import ntt
+    from boojum_cuda_amd.context import Context
+
+    torch.cuda.set_device(0)
+    ctx = Context.create(12, 12)
+    log_n, cols = args.log_n, args.cols
+    n = 1 << log_n
+    g = torch.Generator(device="cuda")
+    g.manual_seed(2)
+    x = uniform_field(cols * n, g)
+    y = torch.empty_like(x)
+    variants = [(False, False), (True, False), (False, True), (True, True)]  # (bitrev_inputs, inverse)
+
+    def step():
+        for br, inv in variants:
+            ntt.batch_ntt_out_of_place(x, y, log_n, cols, 0, 0, n, n, br, inv, 0, 0)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+
+    stream = torch.cuda.current_stream()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2 * args.steps * len(variants))]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    k = 0
+    for _ in range(args.steps):
+        for br, inv in variants:
+            ev[k].record(stream)
+            ntt.batch_ntt_out_of_place(x, y, log_n, cols, 0, 0, n, n, br, inv, 0, 0)
+            ev[k + 1].record(stream)
+            k += 2
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    call_ms = [ev[i].elapsed_time(ev[i + 1]) for i in range(0, k, 2)]
+    avg_call_ms = sum(call_ms) / len(call_ms)
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    elems_per_step = len(variants) * cols * n
+    value = elems_per_step / (ms_per_step * 1e-3)
+    algo_bytes = 16.0 * cols * n
+    achieved = algo_bytes / (avg_call_ms * 1e-3) / 1e9
+    ctx.destroy()
+
+    rec = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "elem/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64",
+        "data": "synthetic",
+        "config": {"workload": f"batched NTT {cols} x 2^{log_n} Goldilocks, fwd/inv x n2b/b2n, out-of-place",
+                   "columns": cols, "log_n": log_n, "calls_per_step": len(variants)},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "one batch_ntt call (all its pass kernels)", "avg_call_ms": avg_call_ms,
+                     "algorithmic_bytes_per_call": algo_bytes},
+    }
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        rec["cpu_baseline"] = cpu_baseline(log_n, args.cpu_cols, threads)
+    print(json.dumps(rec), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,52 @@
+"""Loader for libboojum_ntt.so, the HIP library behind this package.
+
+There is no fallback: if the library is missing, or no GPU is present when a
+transform is issued, the call raises.
+"""
+import ctypes
+import os
+
+_PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(_PKG_ROOT, "libboojum_ntt.so")
+_lib = None
+
+c_u32, c_u64, c_size, c_bool, c_vp, c_int = (ctypes.c_uint32, ctypes.c_uint64, ctypes.c_size_t, ctypes.c_bool,
+                                             ctypes.c_void_p, ctypes.c_int)
+
+# name -> (restype, argtypes); must match include/boojum_ntt.h
+SIGNATURES = {
+    "bc_omega_log_order": (c_u32, []),
+    "bc_context_create": (c_int, [c_u32, c_u32, ctypes.POINTER(c_vp)]),
+    "bc_context_destroy": (c_int, [c_vp]),
+    "bc_batch_ntt_internal": (c_int, [c_vp, c_vp, c_u32, c_u32, c_u32, c_u32, c_bool, c_bool, c_u32, c_u32, c_vp]),
+    "bc_batch_ntt_out_of_place": (c_int, [c_vp, c_size, c_vp, c_size, c_u32, c_u32, c_u32, c_u32, c_u32, c_u32,
+                                          c_bool, c_bool, c_u32, c_u32, c_vp]),
+    "bc_batch_ntt_in_place": (c_int, [c_vp, c_size, c_u32, c_u32, c_u32, c_u32, c_bool, c_bool, c_u32, c_u32,
+                                      c_vp]),
+}
+
+BC_SUCCESS = 0
+BC_ERROR_INVALID_VALUE = 1
+BC_ERROR_NOT_INITIALIZED = 3
+
+
+class HipError(RuntimeError):
+    """A HIP runtime error code returned by the library (CudaResult::Err)."""
+
+    def __init__(self, code, what=""):
+        super().__init__(f"{what} failed with HIP error {code}")
+        self.code = code
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is not built; run __graft_entry__.build() or make -C era-boojum-cuda_amd")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib

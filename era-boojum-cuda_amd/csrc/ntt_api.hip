@@ -1,0 +1,190 @@
+// C ABI entry points: context management and the batched NTT host logic
+// (argument checks, pass planning, launches).  Mirrors src/context.rs and
+// src/ntt.rs of the reference; see include/boojum_ntt.h for the contract.
+#include "../../include/boojum_ntt.h"
+#include "gl_field.hpp"
+#include "ntt_internal.hpp"
+
+#include <mutex>
+#include <vector>
+
+struct bc_context {
+  int device;
+  uint64_t *tables;  // NUM_TABLES * TABLE_LEN u64, layout of bc::Tables
+  bc::Tables tab;
+};
+
+namespace bc {
+namespace {
+
+constexpr int MAX_DEVICES = 64;
+std::mutex g_mu;
+bc_context *g_current[MAX_DEVICES] = {};
+
+// Consecutive powers base^0 .. base^(len-1) (boojum distribute_powers over a
+// vector of ones, as Context::create does at src/context.rs:144-155).
+void fill_powers(uint64_t *dst, uint32_t len, uint64_t base) {
+  uint64_t p = 1;
+  for (uint32_t i = 0; i < len; ++i) {
+    dst[i] = p;
+    p = gl::canon(gl::mul(p, base));
+  }
+}
+
+bc_context *current_context() {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= MAX_DEVICES) return nullptr;
+  std::lock_guard<std::mutex> lk(g_mu);
+  return g_current[dev];
+}
+
+}  // namespace
+}  // namespace bc
+
+extern "C" uint32_t bc_omega_log_order(void) { return bc::OMEGA_LOG_ORDER; }
+
+extern "C" int bc_context_create(uint32_t powers_of_w_coarse_log_count, uint32_t powers_of_g_coarse_log_count,
+                                 bc_context **out) {
+  using namespace bc;
+  if (!out) return BC_ERROR_INVALID_VALUE;
+  *out = nullptr;
+  if (powers_of_w_coarse_log_count > OMEGA_LOG_ORDER || powers_of_g_coarse_log_count > OMEGA_LOG_ORDER)
+    return BC_ERROR_INVALID_VALUE;
+  int dev = 0;
+  hipError_t err = hipGetDevice(&dev);
+  if (err != hipSuccess) return err;
+  if (dev < 0 || dev >= MAX_DEVICES) return BC_ERROR_INVALID_VALUE;
+
+  std::vector<uint64_t> host((size_t)NUM_TABLES * TABLE_LEN);
+  const uint64_t w = gl::OMEGA_2_32;
+  const uint64_t w_coarse = gl::pow(w, TABLE_LEN);
+  const uint64_t g = gl::GENERATOR;
+  const uint64_t gi = gl::inv(g);
+  fill_powers(&host[0 * TABLE_LEN], TABLE_LEN, w);
+  fill_powers(&host[1 * TABLE_LEN], TABLE_LEN, w_coarse);
+  fill_powers(&host[2 * TABLE_LEN], TABLE_LEN, g);
+  fill_powers(&host[3 * TABLE_LEN], TABLE_LEN, gl::pow(g, TABLE_LEN));
+  fill_powers(&host[4 * TABLE_LEN], TABLE_LEN, gi);
+  fill_powers(&host[5 * TABLE_LEN], TABLE_LEN, gl::pow(gi, TABLE_LEN));
+
+  bc_context *ctx = new bc_context();
+  ctx->device = dev;
+  err = hipMalloc((void **)&ctx->tables, host.size() * sizeof(uint64_t));
+  if (err != hipSuccess) {
+    delete ctx;
+    return err;
+  }
+  err = hipMemcpy(ctx->tables, host.data(), host.size() * sizeof(uint64_t), hipMemcpyHostToDevice);
+  if (err != hipSuccess) {
+    (void)hipFree(ctx->tables);
+    delete ctx;
+    return err;
+  }
+  ctx->tab.w_fine = ctx->tables + 0 * TABLE_LEN;
+  ctx->tab.w_coarse = ctx->tables + 1 * TABLE_LEN;
+  ctx->tab.g_fine = ctx->tables + 2 * TABLE_LEN;
+  ctx->tab.g_coarse = ctx->tables + 3 * TABLE_LEN;
+  ctx->tab.gi_fine = ctx->tables + 4 * TABLE_LEN;
+  ctx->tab.gi_coarse = ctx->tables + 5 * TABLE_LEN;
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_current[dev] = ctx;
+  }
+  *out = ctx;
+  return BC_SUCCESS;
+}
+
+extern "C" int bc_context_destroy(bc_context *ctx) {
+  using namespace bc;
+  if (!ctx) return BC_ERROR_INVALID_VALUE;
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (ctx->device >= 0 && ctx->device < MAX_DEVICES && g_current[ctx->device] == ctx)
+      g_current[ctx->device] = nullptr;
+  }
+  hipError_t err = hipFree(ctx->tables);
+  delete ctx;
+  return err;
+}
+
+extern "C" int bc_batch_ntt_internal(const uint64_t *inputs, uint64_t *outputs, uint32_t log_n, uint32_t num_ntts,
+                                     uint32_t stride_in, uint32_t stride_out, bool bitrev_inputs, bool inverse,
+                                     uint32_t log_ext, uint32_t coset_index, void *stream) {
+  using namespace bc;
+  // src/ntt.rs:192-200
+  if (log_n < 1) return BC_ERROR_INVALID_VALUE;
+  if ((uint64_t)log_n + log_ext > OMEGA_LOG_ORDER) return BC_ERROR_INVALID_VALUE;
+  const uint64_t n = 1ull << log_n;
+  if (n > stride_in || n > stride_out) return BC_ERROR_INVALID_VALUE;
+  if (coset_index > 0 && (log_ext == 0 || (uint64_t)coset_index >= (1ull << log_ext))) return BC_ERROR_INVALID_VALUE;
+  if (num_ntts == 0) return BC_SUCCESS;
+  if (!inputs || !outputs) return BC_ERROR_INVALID_VALUE;
+  bc_context *ctx = current_context();
+  if (!ctx) return BC_ERROR_NOT_INITIALIZED;
+
+  uint32_t log_h[8];
+  const int np = plan_passes(log_n, log_h, 8);
+  if (np <= 0) return BC_ERROR_INVALID_VALUE;
+
+  PassArgs a{};
+  a.tab = ctx->tab;
+  a.num_cols = num_ntts;
+  a.log_n = log_n;
+  a.log_ext = log_ext;
+  a.coset_exp = log_ext ? (uint32_t)((uint64_t)coset_index << (32 - log_n - log_ext)) : 0u;
+  a.n_inv = gl::inv(n % gl::P);
+  // block length of each pass in n2b order
+  uint32_t log_N[8];
+  log_N[0] = log_n;
+  for (int p = 1; p < np; ++p) log_N[p] = log_N[p - 1] - log_h[p - 1];
+
+  hipStream_t s = (hipStream_t)stream;
+  for (int k = 0; k < np; ++k) {
+    const int p = bitrev_inputs ? (np - 1 - k) : k;  // b2n runs the passes last-first
+    a.first = k == 0;
+    a.last = k == np - 1;
+    a.in = k == 0 ? inputs : outputs;
+    a.stride_in = k == 0 ? stride_in : stride_out;
+    a.out = outputs;
+    a.stride_out = stride_out;
+    a.log_N = log_N[p];
+    a.log_H = log_h[p];
+    hipError_t err = launch_pass(a, inverse, bitrev_inputs, s);
+    if (err != hipSuccess) return err;
+  }
+  return BC_SUCCESS;
+}
+
+extern "C" int bc_batch_ntt_out_of_place(const uint64_t *inputs_matrix, size_t inputs_len, uint64_t *outputs_matrix,
+                                         size_t outputs_len, uint32_t log_n, uint32_t num_ntts, uint32_t inputs_offset,
+                                         uint32_t outputs_offset, uint32_t stride_in, uint32_t stride_out,
+                                         bool bitrev_inputs, bool inverse, uint32_t log_ext, uint32_t coset_index,
+                                         void *stream) {
+  // src/ntt.rs:309-320, with the products in 64 bits (the reference's u32
+  // products overflow for matrices beyond 2^32 elements).
+  if (!inputs_matrix || !outputs_matrix || log_n >= 32) return BC_ERROR_INVALID_VALUE;
+  const uint64_t n = 1ull << log_n;
+  if ((uint64_t)inputs_offset + n > stride_in) return BC_ERROR_INVALID_VALUE;
+  if ((uint64_t)outputs_offset + n > stride_out) return BC_ERROR_INVALID_VALUE;
+  if ((uint64_t)inputs_len < (uint64_t)num_ntts * stride_in) return BC_ERROR_INVALID_VALUE;
+  if ((uint64_t)outputs_len < (uint64_t)num_ntts * stride_out) return BC_ERROR_INVALID_VALUE;
+  if ((const void *)inputs_matrix == (const void *)outputs_matrix) return BC_ERROR_INVALID_VALUE;
+  const uint64_t *in = inputs_matrix + inputs_offset;
+  uint64_t *out = outputs_matrix + outputs_offset;
+  if ((const void *)in == (const void *)out) return BC_ERROR_INVALID_VALUE;
+  return bc_batch_ntt_internal(in, out, log_n, num_ntts, stride_in, stride_out, bitrev_inputs, inverse, log_ext,
+                               coset_index, stream);
+}
+
+extern "C" int bc_batch_ntt_in_place(uint64_t *inputs_matrix, size_t inputs_len, uint32_t log_n, uint32_t num_ntts,
+                                     uint32_t inputs_offset, uint32_t stride, bool bitrev_inputs, bool inverse,
+                                     uint32_t log_ext, uint32_t coset_index, void *stream) {
+  // src/ntt.rs:350-352
+  if (!inputs_matrix || log_n >= 32) return BC_ERROR_INVALID_VALUE;
+  const uint64_t n = 1ull << log_n;
+  if ((uint64_t)inputs_offset + n > stride) return BC_ERROR_INVALID_VALUE;
+  if ((uint64_t)inputs_len < (uint64_t)num_ntts * stride) return BC_ERROR_INVALID_VALUE;
+  uint64_t *p = inputs_matrix + inputs_offset;
+  return bc_batch_ntt_internal(p, p, log_n, num_ntts, stride, stride, bitrev_inputs, inverse, log_ext, coset_index,
+                               stream);
+}

@@ -1,0 +1,60 @@
+// Shared declarations between the C-ABI layer and the NTT kernels.
+#pragma once
+#include <stdint.h>
+#include <hip/hip_runtime.h>
+
+namespace bc {
+
+// Largest supported log2 transform length (including the LDE blowup).
+// The reference caps this at 24 (src/context.rs:12); the power tables here
+// cover the whole 2^32 two-adic subgroup.
+constexpr uint32_t OMEGA_LOG_ORDER = 32;
+
+// Power tables resident in HBM (one allocation, built by bc_context_create):
+//   w_fine[i]   = omega_{2^32}^i            i < 2^16
+//   w_coarse[i] = omega_{2^32}^(i * 2^16)   i < 2^16
+//   g_fine / g_coarse: powers of g = 7, same split; gi_*: powers of g^-1.
+// omega_{2^32}^E for any u32 exponent E is w_coarse[E >> 16] * w_fine[E & 0xFFFF],
+// so every omega_{2^k} power is an exponent E = e << (32 - k) in u32 arithmetic
+// (wrap-around is reduction mod 2^32, i.e. mod the subgroup order).
+struct Tables {
+  const uint64_t *w_fine;
+  const uint64_t *w_coarse;
+  const uint64_t *g_fine;
+  const uint64_t *g_coarse;
+  const uint64_t *gi_fine;
+  const uint64_t *gi_coarse;
+};
+constexpr uint32_t TABLE_LOG = 16;
+constexpr uint32_t TABLE_LEN = 1u << TABLE_LOG;
+constexpr uint32_t NUM_TABLES = 6;
+
+// One HBM pass of a batched transform (see ntt_kernels.hip for the math).
+struct PassArgs {
+  const uint64_t *in;
+  uint64_t *out;
+  uint64_t stride_in;   // elements between consecutive input columns
+  uint64_t stride_out;  // elements between consecutive output columns
+  uint64_t n_inv;       // n^-1 mod p (last pass of an inverse transform)
+  Tables tab;
+  uint32_t num_cols;
+  uint32_t log_n;      // column length
+  uint32_t log_N;      // block length this pass works inside
+  uint32_t log_H;      // sub-transform length this pass computes
+  uint32_t first;      // pass reads the caller's input (coset pre-scaling)
+  uint32_t last;       // pass writes the final output (inverse post-scaling)
+  uint32_t log_ext;    // LDE: log2 blowup, 0 = plain transform
+  uint32_t coset_exp;  // coset_index << (32 - log_n - log_ext)
+};
+
+constexpr uint32_t TILE_LOG = 12;
+constexpr uint32_t TILE = 1u << TILE_LOG;
+constexpr uint32_t TILE_THREADS = 256;
+
+// Host: the split of log_n into per-pass sub-transform sizes, n2b order
+// (first entry = the pass that works on the top index bits).
+int plan_passes(uint32_t log_n, uint32_t *log_h, int max_passes);
+
+hipError_t launch_pass(const PassArgs &a, bool inverse, bool b2n, hipStream_t stream);
+
+}  // namespace bc

@@ -1,0 +1,79 @@
+/*
+ * C ABI of the MI355X (gfx950) batched Goldilocks NTT / coset-LDE library
+ * (libboojum_ntt.so).  This is the drop-in boundary for boojum-cuda's `ntt`
+ * and `context` modules: a Rust FFI crate (or any C caller) binds exactly
+ * these symbols in place of the CUDA kernels that era_cudart launched.
+ *
+ * Conventions (same as the reference):
+ *   - Elements are Goldilocks field elements stored as uint64_t, any value
+ *     in [0, 2^64) is accepted as its residue mod p = 2^64 - 2^32 + 1.
+ *     Outputs are canonical, in [0, p).
+ *   - Matrices are column-major: column i starts at base + i * stride.
+ *   - All transforms are asynchronous on `stream` (a hipStream_t; NULL is the
+ *     null stream) and allocate nothing.
+ *   - One context per process and device must exist (bc_context_create)
+ *     before a transform is issued, as the reference requires a live
+ *     Context (src/context.rs:157-282).
+ *   - Return value: 0 on success; where the reference panics on an argument
+ *     assert this returns BC_ERROR_INVALID_VALUE before anything is launched;
+ *     launch failures return the HIP error code.
+ */
+#ifndef BOOJUM_NTT_H
+#define BOOJUM_NTT_H
+
+#include <stdbool.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BC_SUCCESS 0
+#define BC_ERROR_INVALID_VALUE 1   /* == hipErrorInvalidValue */
+#define BC_ERROR_NOT_INITIALIZED 3 /* == hipErrorNotInitialized: no context */
+
+typedef struct bc_context bc_context;
+
+/* OMEGA_LOG_ORDER (reference: src/context.rs:12, value 24 there; 32 here so
+ * log_n + log_extension_degree may reach the full two-adic subgroup). */
+uint32_t bc_omega_log_order(void);
+
+/* Context::create(powers_of_w_coarse_log_count, powers_of_g_coarse_log_count)
+ * (reference: src/context.rs:170-282).  Builds the power tables on the
+ * current device and makes them the process-wide tables for that device.
+ * Both arguments must be <= bc_omega_log_order() (src/context.rs:175-176);
+ * the fine/coarse split of this implementation is fixed (2^16 x 2^16). */
+int bc_context_create(uint32_t powers_of_w_coarse_log_count, uint32_t powers_of_g_coarse_log_count,
+                      bc_context **out);
+
+/* Context::destroy (reference: src/context.rs:284-296). */
+int bc_context_destroy(bc_context *ctx);
+
+/* batch_ntt_internal (reference: src/ntt.rs:179-291).  Pointers are already
+ * offset; strides are in elements.  Asserts of src/ntt.rs:192-200. */
+int bc_batch_ntt_internal(const uint64_t *inputs, uint64_t *outputs, uint32_t log_n, uint32_t num_ntts,
+                          uint32_t stride_between_input_arrays, uint32_t stride_between_output_arrays,
+                          bool bitrev_inputs, bool inverse, uint32_t log_extension_degree, uint32_t coset_index,
+                          void *stream);
+
+/* batch_ntt_out_of_place (reference: src/ntt.rs:293-335).  `*_len` are the
+ * DeviceSlice lengths in elements (checked as at src/ntt.rs:310-313, in 64-bit
+ * arithmetic); the matrices must not start at the same address
+ * (src/ntt.rs:317,320). */
+int bc_batch_ntt_out_of_place(const uint64_t *inputs_matrix, size_t inputs_len, uint64_t *outputs_matrix,
+                              size_t outputs_len, uint32_t log_n, uint32_t num_ntts, uint32_t inputs_offset,
+                              uint32_t outputs_offset, uint32_t stride_between_input_arrays,
+                              uint32_t stride_between_output_arrays, bool bitrev_inputs, bool inverse,
+                              uint32_t log_extension_degree, uint32_t coset_index, void *stream);
+
+/* batch_ntt_in_place (reference: src/ntt.rs:337-369). */
+int bc_batch_ntt_in_place(uint64_t *inputs_matrix, size_t inputs_len, uint32_t log_n, uint32_t num_ntts,
+                          uint32_t inputs_offset, uint32_t stride_between_input_arrays, bool bitrev_inputs,
+                          bool inverse, uint32_t log_extension_degree, uint32_t coset_index, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BOOJUM_NTT_H */

@@ -1,0 +1,187 @@
+"""ctypes wrapper around the CPU oracle (oracle/gl_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, never by the product package.  The C file's
+header lists the reference call sites (file:line) each function restates.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+P = 0xFFFFFFFF00000001
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle_gl.so")
+_lib = None
+
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+
+
+def build():
+    """Compile the oracle with gcc (seconds)."""
+    subprocess.check_call(["make", "-s", "-C", _HERE, "liboracle_gl.so"])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        u64, u32, i32 = ctypes.c_uint64, ctypes.c_uint32, ctypes.c_int
+        sig = {
+            "gl_add": (u64, [u64, u64]),
+            "gl_sub": (u64, [u64, u64]),
+            "gl_mul": (u64, [u64, u64]),
+            "gl_pow": (u64, [u64, u64]),
+            "gl_inv": (u64, [u64]),
+            "gl_generator": (u64, []),
+            "gl_omega_2_32": (u64, []),
+            "gl_domain_generator": (u64, [u32]),
+            "gl_bitrev": (u64, [u64, u32]),
+            "gl_bitreverse_inplace": (None, [_u64p, u64]),
+            "gl_distribute_powers": (None, [_u64p, u64, u64]),
+            "gl_precompute_twiddles": (None, [u64, i32, _u64p]),
+            "gl_serial_ct_n2b": (None, [_u64p, u64, _u64p]),
+            "gl_fft_natural_to_bitreversed": (None, [_u64p, u64, u64, _u64p]),
+            "gl_ifft_natural_to_natural": (None, [_u64p, u64, u64, _u64p]),
+            "gl_coset_shift": (u64, [u32, u32, u32]),
+            "gl_ntt_column": (None, [_u64p, _u64p, u32, i32, i32, u32, u32, _u64p]),
+            "gl_batch_ntt": (None, [_u64p, _u64p, u32, u32, u64, u64, i32, i32, u32, u32]),
+            "gl_batch_ntt_mt": (i32, [_u64p, _u64p, u32, u32, u64, u64, i32, i32, u32, u32, u32]),
+            "gl_dft_naive": (None, [_u64p, _u64p, u64, u64]),
+            "gl_horner": (u64, [_u64p, u64, u64]),
+            "gl_fill_uniform": (None, [_u64p, u64, u64, u64]),
+            "gl_fill_raw": (None, [_u64p, u64, u64, u64]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    assert a.dtype == np.uint64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_u64p)
+
+
+# ---- scalars -------------------------------------------------------------
+
+def add(a, b):
+    return lib().gl_add(a, b)
+
+
+def sub(a, b):
+    return lib().gl_sub(a, b)
+
+
+def mul(a, b):
+    return lib().gl_mul(a, b)
+
+
+def pow_(b, e):
+    return lib().gl_pow(b, e)
+
+
+def inv(a):
+    return lib().gl_inv(a)
+
+
+def generator():
+    return lib().gl_generator()
+
+
+def omega_2_32():
+    return lib().gl_omega_2_32()
+
+
+def domain_generator(log_size):
+    return lib().gl_domain_generator(log_size)
+
+
+def coset_shift(log_n, log_ext, coset_index):
+    return lib().gl_coset_shift(log_n, log_ext, coset_index)
+
+
+def bitrev(x, bits):
+    return lib().gl_bitrev(x, bits)
+
+
+# ---- arrays ----------------------------------------------------------------
+
+def fill_uniform(n, seed, column=0):
+    out = np.empty(n, dtype=np.uint64)
+    lib().gl_fill_uniform(_ptr(out), n, seed, column)
+    return out
+
+
+def fill_raw(n, seed, column=0):
+    out = np.empty(n, dtype=np.uint64)
+    lib().gl_fill_raw(_ptr(out), n, seed, column)
+    return out
+
+
+def matrix_uniform(num_cols, n, seed, stride=None, offset=0, raw=False):
+    """Column-major matrix: column c occupies [c*stride + offset, +n)."""
+    stride = n if stride is None else stride
+    m = np.zeros(num_cols * stride, dtype=np.uint64)
+    for c in range(num_cols):
+        col = fill_raw(n, seed, c) if raw else fill_uniform(n, seed, c)
+        m[c * stride + offset: c * stride + offset + n] = col
+    return m
+
+
+def bitreverse(a):
+    a = np.ascontiguousarray(a, dtype=np.uint64).copy()
+    lib().gl_bitreverse_inplace(_ptr(a), a.size)
+    return a
+
+
+def twiddles(n, inverse):
+    out = np.zeros(max(n // 2, 1), dtype=np.uint64)
+    lib().gl_precompute_twiddles(n, int(inverse), _ptr(out))
+    return out[: n // 2]
+
+
+def ntt_column(x, log_n, bitrev_inputs, inverse, log_ext=0, coset_index=0):
+    x = np.ascontiguousarray(x, dtype=np.uint64)
+    assert x.size == 1 << log_n
+    out = np.empty_like(x)
+    lib().gl_ntt_column(_ptr(x), _ptr(out), log_n, int(bitrev_inputs), int(inverse), log_ext, coset_index, None)
+    return out
+
+
+def batch_ntt(inp, log_n, num_ntts, stride_in, stride_out, bitrev_inputs, inverse, log_ext=0, coset_index=0,
+              out=None, threads=1):
+    """Batched oracle over flat column-major buffers (offsets pre-applied by slicing)."""
+    inp = np.ascontiguousarray(inp, dtype=np.uint64)
+    if out is None:
+        out = np.zeros(max(inp.size, (num_ntts - 1) * stride_out + (1 << log_n)), dtype=np.uint64)
+    if threads > 1:
+        lib().gl_batch_ntt_mt(_ptr(inp), _ptr(out), log_n, num_ntts, stride_in, stride_out, int(bitrev_inputs),
+                              int(inverse), log_ext, coset_index, threads)
+    else:
+        lib().gl_batch_ntt(_ptr(inp), _ptr(out), log_n, num_ntts, stride_in, stride_out, int(bitrev_inputs),
+                           int(inverse), log_ext, coset_index)
+    return out
+
+
+def dft_naive(a, root):
+    a = np.ascontiguousarray(a, dtype=np.uint64)
+    out = np.empty_like(a)
+    lib().gl_dft_naive(_ptr(a), _ptr(out), a.size, root)
+    return out
+
+
+def horner(a, x):
+    a = np.ascontiguousarray(a, dtype=np.uint64)
+    return lib().gl_horner(_ptr(a), a.size, x)
+
+
+def canon(a):
+    """Reduce a uint64 array to canonical representatives in [0, p)."""
+    a = np.asarray(a, dtype=np.uint64)
+    return np.where(a >= np.uint64(P), a - np.uint64(P), a)
