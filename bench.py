@@ -62,6 +62,8 @@ def main():
     ap.add_argument("--cols", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-cols", type=int, default=64)
+    ap.add_argument("--mode", choices=["ntt", "copy"], default="ntt",
+                    help="copy: time a plain device copy of the same matrix (calibrates HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (columns shard with no "
@@ -80,6 +82,19 @@ def main():
     g.manual_seed(2)
     x = uniform_field(cols * n, g)
     y = torch.empty_like(x)
+    if args.mode == "copy":
+        for _ in range(args.warmup):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / args.steps
+        print(json.dumps({"mode": "copy", "bytes_per_copy": 16 * cols * n, "ms": dt * 1e3,
+                          "GBs": 16 * cols * n / dt / 1e9}), flush=True)
+        ctx.destroy()
+        return
     variants = [(False, False), (True, False), (False, True), (True, True)]  # (bitrev_inputs, inverse)
 
     def step():

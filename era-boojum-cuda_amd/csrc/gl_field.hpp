@@ -77,13 +77,24 @@ GL_FN uint64_t mul(uint64_t a, uint64_t b) {
   return reduce128(lo, hi);
 }
 
-// x * 2^E mod p for a compile-time exponent 0 <= E < 192 (2 has order 192).
+// Reduce lo + hi * 2^64 for hi < 2^32: hi * 2^64 = hi * (2^32 - 1).
+GL_FN uint64_t reduce96(uint64_t lo, uint32_t hi) {
+  const uint64_t u = ((uint64_t)hi << 32) - hi;
+  uint64_t r;
+  bool c = __builtin_add_overflow(lo, u, &r);
+  // lo + u - 2^64 + EPS < u + EPS < 2^64: no second wrap.
+  return r + (c ? EPS : 0ull);
+}
+
+// x * 2^E mod p for a compile-time exponent 0 <= E < 96.  (2 has order 192
+// and 2^96 = -1; callers fold the sign of larger exponents into the
+// surrounding add/sub.)
 template <unsigned E> GL_FN uint64_t mul_pow2(uint64_t x) {
-  static_assert(E < 192, "2 has multiplicative order 192 mod p");
+  static_assert(E < 96, "use the sign-folded form for exponents >= 96");
   if constexpr (E == 0) {
     return x;
-  } else if constexpr (E >= 96) {
-    return neg(mul_pow2<E - 96>(x));  // 2^96 = -1
+  } else if constexpr (E <= 32) {
+    return reduce96(x << E, (uint32_t)(x >> (64 - E)));
   } else if constexpr (E < 64) {
     return reduce128(x << E, x >> (64 - E));
   } else if constexpr (E == 64) {
@@ -95,6 +106,12 @@ template <unsigned E> GL_FN uint64_t mul_pow2(uint64_t x) {
     const uint64_t hi = x >> (128 - E);
     return sub(reduce128(0ull, lo), hi << 32);
   }
+}
+
+// x * 2^E for any 0 <= E < 192.
+template <unsigned E> GL_FN uint64_t mul_pow2_any(uint64_t x) {
+  if constexpr (E >= 96) return neg(mul_pow2<E - 96>(x));
+  else return mul_pow2<E>(x);
 }
 
 GL_FN uint64_t pow(uint64_t b, uint64_t e) {
@@ -117,7 +134,7 @@ constexpr uint64_t GENERATOR = 7;
 constexpr uint64_t OMEGA_2_32 = 0x185629DCDA58878Cull;
 constexpr unsigned OMEGA_64_LOG2 = 39;
 
-// Exponent e with omega_{2^a}^t = 2^e (forward) or its inverse.
+// Exponent e in [0, 192) with omega_{2^a}^t = 2^e (forward) or its inverse.
 constexpr unsigned root_pow2_exp(unsigned a, unsigned t, bool inverse) {
   // omega_{2^a} = omega_64^(2^(6-a)) = 2^(39 * 2^(6-a)).
   unsigned base = (OMEGA_64_LOG2 << (6 - a)) % 192u;

@@ -38,6 +38,60 @@ bc_context *current_context() {
   return g_current[dev];
 }
 
+
+// Split of log_n >= 8 into strided passes of 4..8 bits (top index bits first)
+// and one contiguous pass of 8..12 bits.
+struct FastPlan {
+  int num_strided;
+  uint32_t strided[8];
+  uint32_t log_N[8];  // block length of each strided pass
+  uint32_t contig;
+};
+
+void make_fast_plan(uint32_t log_n, FastPlan &pl) {
+  pl.num_strided = 0;
+  if (log_n <= TILE_LOG) {
+    pl.contig = log_n;
+    return;
+  }
+  uint32_t c = TILE_LOG, s = log_n - TILE_LOG;
+  const uint32_t k = (s + 7) / 8;
+  if (s < 4 * k) {  // only when k == 1 and s < 4: shrink the contiguous pass
+    c -= 4 * k - s;
+    s = 4 * k;
+  }
+  uint32_t N = log_n;
+  for (uint32_t i = 0; i < k; ++i) {
+    const uint32_t part = s / k + (i < s % k ? 1 : 0);
+    pl.strided[i] = part;
+    pl.log_N[i] = N;
+    N -= part;
+  }
+  pl.num_strided = (int)k;
+  pl.contig = c;
+}
+
+// log_n < 8: one pass of the generic LDS kernel.
+int run_general(bc_context *ctx, const uint64_t *inputs, uint64_t *outputs, uint32_t log_n, uint32_t num_ntts,
+                uint32_t stride_in, uint32_t stride_out, bool bitrev_inputs, bool inverse, uint32_t log_ext,
+                uint32_t coset_exp, uint64_t n_inv, hipStream_t s) {
+  PassArgs a{};
+  a.tab = ctx->tab;
+  a.num_cols = num_ntts;
+  a.log_n = log_n;
+  a.log_ext = log_ext;
+  a.coset_exp = coset_exp;
+  a.n_inv = n_inv;
+  a.first = a.last = 1;
+  a.in = inputs;
+  a.stride_in = stride_in;
+  a.out = outputs;
+  a.stride_out = stride_out;
+  a.log_N = log_n;
+  a.log_H = log_n;
+  return launch_pass(a, inverse, bitrev_inputs, s);
+}
+
 }  // namespace
 }  // namespace bc
 
@@ -122,34 +176,64 @@ extern "C" int bc_batch_ntt_internal(const uint64_t *inputs, uint64_t *outputs, 
   bc_context *ctx = current_context();
   if (!ctx) return BC_ERROR_NOT_INITIALIZED;
 
-  uint32_t log_h[8];
-  const int np = plan_passes(log_n, log_h, 8);
-  if (np <= 0) return BC_ERROR_INVALID_VALUE;
+  hipStream_t s = (hipStream_t)stream;
+  const uint32_t coset_exp = log_ext ? (uint32_t)((uint64_t)coset_index << (32 - log_n - log_ext)) : 0u;
+  const uint64_t n_inv = gl::inv(n % gl::P);
 
-  PassArgs a{};
+  if (log_n < 8) return run_general(ctx, inputs, outputs, log_n, num_ntts, stride_in, stride_out, bitrev_inputs,
+                                    inverse, log_ext, coset_exp, n_inv, s);
+
+  FastPlan plan;
+  make_fast_plan(log_n, plan);
+  FastArgs a{};
   a.tab = ctx->tab;
   a.num_cols = num_ntts;
   a.log_n = log_n;
   a.log_ext = log_ext;
-  a.coset_exp = log_ext ? (uint32_t)((uint64_t)coset_index << (32 - log_n - log_ext)) : 0u;
-  a.n_inv = gl::inv(n % gl::P);
-  // block length of each pass in n2b order
-  uint32_t log_N[8];
-  log_N[0] = log_n;
-  for (int p = 1; p < np; ++p) log_N[p] = log_N[p - 1] - log_h[p - 1];
-
-  hipStream_t s = (hipStream_t)stream;
+  a.coset_exp = coset_exp;
+  a.n_inv = n_inv;
+  const int np = plan.num_strided + 1;
+  const bool single = np == 1;
   for (int k = 0; k < np; ++k) {
-    const int p = bitrev_inputs ? (np - 1 - k) : k;  // b2n runs the passes last-first
-    a.first = k == 0;
-    a.last = k == np - 1;
-    a.in = k == 0 ? inputs : outputs;
-    a.stride_in = k == 0 ? stride_in : stride_out;
+    // n2b runs strided passes top-down then the contiguous pass; b2n the reverse
+    const int p = bitrev_inputs ? (np - 1 - k) : k;  // index in n2b order
+    const bool first = k == 0, last = k == np - 1;
+    a.in = first ? inputs : outputs;
+    a.stride_in = first ? stride_in : stride_out;
     a.out = outputs;
     a.stride_out = stride_out;
-    a.log_N = log_N[p];
-    a.log_H = log_h[p];
-    hipError_t err = launch_pass(a, inverse, bitrev_inputs, s);
+    a.last = last;
+    hipError_t err;
+    if (p == np - 1) {
+      int mode = MODE_NONE;
+      if (!inverse && log_ext && first) mode = MODE_LDE_IN;
+      else if (inverse && last && log_ext) mode = MODE_LDE_OUT;
+      else if (inverse && last && single) mode = MODE_NINV;
+      a.log_N = plan.contig;
+      const uint64_t tiles = ((uint64_t)num_ntts << log_n) >> TILE_LOG;
+      const uint64_t tiles_ceil = log_n >= TILE_LOG ? tiles : (num_ntts + (1u << (TILE_LOG - log_n)) - 1) >> (TILE_LOG - log_n);
+      if (tiles_ceil > 0xFFFFFFFFull) return BC_ERROR_INVALID_VALUE;
+      a.tiles = (uint32_t)tiles_ceil;
+      err = launch_contig(a, plan.contig, inverse, bitrev_inputs, mode, s);
+    } else {
+      const bool top = p == 0;
+      int mode = MODE_NONE;
+      if (top) {
+        if (!inverse && log_ext && !bitrev_inputs) mode = MODE_LDE_IN;
+        else if (inverse && log_ext && bitrev_inputs) mode = MODE_LDE_OUT;
+        else if (inverse && !log_ext) mode = MODE_NINV;
+      }
+      const uint32_t lh = plan.strided[p];
+      a.log_N = plan.log_N[p];
+      const uint64_t positions = 1ull << (a.log_N - TILE_LOG);
+      const uint64_t pairs = (uint64_t)num_ntts << (log_n - a.log_N);
+      uint64_t bpw = pairs * positions / 2048;
+      bpw = bpw < 1 ? 1 : (bpw > 16 ? 16 : bpw);
+      const uint64_t groups = (pairs + bpw - 1) / bpw;
+      if (positions > 0x7FFFFFFFull || groups > 65535) return BC_ERROR_INVALID_VALUE;
+      a.blocks_per_wg = (uint32_t)bpw;
+      err = launch_strided(a, (int)lh, inverse, bitrev_inputs, mode, (uint32_t)positions, (uint32_t)groups, s);
+    }
     if (err != hipSuccess) return err;
   }
   return BC_SUCCESS;

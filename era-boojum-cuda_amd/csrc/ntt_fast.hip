@@ -1,0 +1,425 @@
+// Register-resident pass kernels for the batched NTT on gfx950.
+//
+// The math of a pass is the one described in ntt_kernels.hip: inside blocks of
+// N points, for every r < R = N / H, an H-point sub-DFT over j (positions
+// j * R + r), with the twist omega_N^(r * brev(j)).  Two tile shapes cover
+// every pass of a transform with log_n >= 8:
+//
+//   contig<h>  (8 <= h <= 12): R = 1, the last pass in n2b order.  A tile is
+//              4096 consecutive points = 2^(12-h) sub-DFTs.  Levels of radix
+//              16, 16 and 2^(h-8) run in registers, two LDS exchanges.
+//   strided<h> (4 <= h <= 8):  R >= 16, every other pass.  A tile is
+//              2^(12-h) consecutive r times the h-point sub-DFT, so each row of
+//              the tile is >= 128 contiguous bytes.  Levels of radix 16 and
+//              2^(h-4), one LDS exchange (none for h = 4).
+//
+// Every workgroup has 256 threads holding 16 values each.  Global loads go
+// straight into the first level's registers and the last level stores from
+// registers.  Per-thread level twists depend only on the thread and are
+// computed once per workgroup; the strided pass twist depends on the tile's r
+// range, so a workgroup walks several (column, block) pairs that share it and
+// computes it once.  n2b runs levels top-down (DIF), b2n bottom-up (DIT) as
+// the transpose.
+#include "ntt_device.hpp"
+
+namespace bc {
+namespace {
+
+constexpr uint32_t NT = 256;  // threads per workgroup
+
+// Padded LDS index: one pad slot per 16 keeps stride-16 accesses on distinct banks.
+__device__ __forceinline__ uint32_t pidx(uint32_t e) { return e + (e >> 4); }
+constexpr uint32_t LDS_PADDED = TILE + TILE / 16;
+
+template <int N> __device__ __forceinline__ void twist_mul(uint64_t *x, const uint64_t *tw) {
+#pragma unroll
+  for (int i = 1; i < N; ++i) x[i] = gl::mul(x[i], tw[i - 1]);
+}
+
+// Per-thread level twist omega_M^(rp * brev4(i)), i = 1..15, M = 2^log_m.
+template <bool INV>
+__device__ __forceinline__ void load_level_twists(const Tables &tab, uint32_t rp, uint32_t log_m, uint64_t *tw) {
+#pragma unroll
+  for (int i = 1; i < 16; ++i) {
+    uint32_t e = (rp * (uint32_t)(__brev(i) >> 28)) << (32u - log_m);
+    if (INV) e = 0u - e;
+    tw[i - 1] = pow_w_coarse(tab, e);
+  }
+}
+
+// -------------------------------------------------------------------------
+// contig<h>
+
+// Tile element e of thread t's group g, point i, at level L (width A) of an
+// h-point sub-DFT whose levels are 4, 4, h - 8.
+template <int LH, int L, int A> __device__ __forceinline__ uint32_t celem(uint32_t t, int g, int i) {
+  constexpr int C = 4 * L, LM = LH - C, LS = LM - A;
+  const uint32_t gi = t + NT * (uint32_t)g;
+  const uint32_t rp = gi & ((1u << LS) - 1u);
+  const uint32_t b = (gi >> LS) & ((1u << C) - 1u);
+  const uint32_t qb = gi >> (LH - A);
+  return (qb << LH) + (b << LM) + ((uint32_t)i << LS) + rp;
+}
+
+// Scaling factor of a contiguous-pass element at column position pos.
+template <int MODE, bool DIT>
+__device__ __forceinline__ uint64_t contig_scale(const FastArgs &a, uint32_t pos) {
+  if constexpr (MODE == MODE_LDE_IN) {
+    const uint32_t idx = DIT ? brev_bits(pos, a.log_n) : pos;  // input position holds x_idx
+    return coset_pow(a.tab, a.coset_exp, idx, false);
+  } else {
+    const uint32_t idx = DIT ? pos : brev_bits(pos, a.log_n);  // output position holds y_idx
+    return gl::mul(a.n_inv, coset_pow(a.tab, a.coset_exp, idx, true));
+  }
+}
+
+template <int LH, bool INV, bool DIT, int MODE>
+__global__ __launch_bounds__(NT) void ntt_contig_kernel(FastArgs a) {
+  constexpr int NL = LH > 8 ? 3 : 2;
+  constexpr int A2 = NL == 3 ? LH - 8 : 4;  // width of the third level
+  constexpr int G2 = 16 >> A2;              // third-level groups per thread
+  constexpr int SUB = 12 - LH;              // log2 sub-DFTs per tile
+  __shared__ uint64_t lds[LDS_PADDED];
+  const uint32_t t = threadIdx.x;
+
+  uint64_t tw0[15], tw1[15];
+  load_level_twists<INV>(a.tab, t & ((1u << (LH - 4)) - 1u), LH, tw0);
+  if constexpr (NL == 3) load_level_twists<INV>(a.tab, t & ((1u << (LH - 8)) - 1u), LH - 4, tw1);
+
+  const bool multicol = a.log_n < TILE_LOG;  // tile spans 2^SUB columns (single-pass case)
+  for (uint32_t tile = blockIdx.x; tile < a.tiles; tile += gridDim.x) {
+    // Address of tile element e: in-column tiles are 4096 contiguous points.
+    uint64_t base_in, base_out;
+    uint32_t pos0 = 0, col0 = 0;
+    if (!multicol) {
+      const uint32_t sh = a.log_n - TILE_LOG;
+      const uint64_t col = tile >> sh;
+      pos0 = (tile & ((1u << sh) - 1u)) << TILE_LOG;
+      base_in = col * a.stride_in + pos0;
+      base_out = col * a.stride_out + pos0;
+    } else {
+      col0 = tile << SUB;
+      base_in = base_out = 0;
+    }
+    auto addr_in = [&](uint32_t e, bool &ok) -> uint64_t {
+      if (!multicol) { ok = true; return base_in + e; }
+      const uint32_t c = col0 + (e >> LH);
+      ok = c < a.num_cols;
+      return (uint64_t)c * a.stride_in + (e & ((1u << LH) - 1u));
+    };
+    auto addr_out = [&](uint32_t e, bool &ok) -> uint64_t {
+      if (!multicol) { ok = true; return base_out + e; }
+      const uint32_t c = col0 + (e >> LH);
+      ok = c < a.num_cols;
+      return (uint64_t)c * a.stride_out + (e & ((1u << LH) - 1u));
+    };
+    auto pos_of = [&](uint32_t e) -> uint32_t { return multicol ? (e & ((1u << LH) - 1u)) : pos0 + e; };
+
+    uint64_t x[16];
+    if constexpr (!DIT) {
+      // level 0 straight from HBM (lanes walk consecutive points)
+#pragma unroll
+      for (int i = 0; i < 16; ++i) {
+        bool ok;
+        const uint64_t ad = addr_in(celem<LH, 0, 4>(t, 0, i), ok);
+        x[i] = ok ? a.in[ad] : 0ull;
+      }
+      if constexpr (MODE == MODE_LDE_IN) {
+#pragma unroll
+        for (int i = 0; i < 16; ++i) x[i] = gl::mul(x[i], contig_scale<MODE, DIT>(a, pos_of(celem<LH, 0, 4>(t, 0, i))));
+      }
+      dft_dif<4, INV>(x);
+      twist_mul<16>(x, tw0);
+#pragma unroll
+      for (int i = 0; i < 16; ++i) lds[pidx(celem<LH, 0, 4>(t, 0, i))] = x[i];
+      __syncthreads();
+#pragma unroll
+      for (int i = 0; i < 16; ++i) x[i] = lds[pidx(celem<LH, 1, 4>(t, 0, i))];
+      dft_dif<4, INV>(x);
+      if constexpr (NL == 3) {
+        twist_mul<16>(x, tw1);
+#pragma unroll
+        for (int i = 0; i < 16; ++i) lds[pidx(celem<LH, 1, 4>(t, 0, i))] = x[i];
+        __syncthreads();
+#pragma unroll
+        for (int g = 0; g < G2; ++g)
+#pragma unroll
+          for (int i = 0; i < (1 << A2); ++i) x[g * (1 << A2) + i] = lds[pidx(celem<LH, 2, A2>(t, g, i))];
+#pragma unroll
+        for (int g = 0; g < G2; ++g) dft_dif<A2, INV>(x + g * (1 << A2));
+      }
+      // store from the last level's registers
+      constexpr int LL = NL - 1, AL = NL == 3 ? A2 : 4, GL = 16 >> AL;
+#pragma unroll
+      for (int g = 0; g < GL; ++g)
+#pragma unroll
+        for (int i = 0; i < (1 << AL); ++i) {
+          const uint32_t e = celem<LH, LL, AL>(t, g, i);
+          uint64_t v = x[g * (1 << AL) + i];
+          if constexpr (MODE == MODE_NINV) v = gl::mul(v, a.n_inv);
+          if constexpr (MODE == MODE_LDE_OUT) v = gl::mul(v, contig_scale<MODE, DIT>(a, pos_of(e)));
+          bool ok;
+          const uint64_t ad = addr_out(e, ok);
+          if (ok) a.out[ad] = gl::canon(v);  // n2b: the contiguous pass is the last one
+        }
+    } else {
+      constexpr int LL = NL - 1, AL = NL == 3 ? A2 : 4, GL = 16 >> AL;
+#pragma unroll
+      for (int g = 0; g < GL; ++g)
+#pragma unroll
+        for (int i = 0; i < (1 << AL); ++i) {
+          const uint32_t e = celem<LH, LL, AL>(t, g, i);
+          bool ok;
+          const uint64_t ad = addr_in(e, ok);
+          uint64_t v = ok ? a.in[ad] : 0ull;
+          if constexpr (MODE == MODE_LDE_IN) v = gl::mul(v, contig_scale<MODE, DIT>(a, pos_of(e)));
+          x[g * (1 << AL) + i] = v;
+        }
+      if constexpr (NL == 3) {
+#pragma unroll
+        for (int g = 0; g < G2; ++g) dft_dit<A2, INV>(x + g * (1 << A2));
+#pragma unroll
+        for (int g = 0; g < G2; ++g)
+#pragma unroll
+          for (int i = 0; i < (1 << A2); ++i) lds[pidx(celem<LH, 2, A2>(t, g, i))] = x[g * (1 << A2) + i];
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < 16; ++i) x[i] = lds[pidx(celem<LH, 1, 4>(t, 0, i))];
+        twist_mul<16>(x, tw1);
+      }
+      dft_dit<4, INV>(x);
+#pragma unroll
+      for (int i = 0; i < 16; ++i) lds[pidx(celem<LH, 1, 4>(t, 0, i))] = x[i];
+      __syncthreads();
+#pragma unroll
+      for (int i = 0; i < 16; ++i) x[i] = lds[pidx(celem<LH, 0, 4>(t, 0, i))];
+      twist_mul<16>(x, tw0);
+      dft_dit<4, INV>(x);
+#pragma unroll
+      for (int i = 0; i < 16; ++i) {
+        const uint32_t e = celem<LH, 0, 4>(t, 0, i);
+        uint64_t v = x[i];
+        if constexpr (MODE == MODE_NINV) v = gl::mul(v, a.n_inv);
+        if constexpr (MODE == MODE_LDE_OUT) v = gl::mul(v, contig_scale<MODE, DIT>(a, pos_of(e)));
+        if (a.last) v = gl::canon(v);
+        bool ok;
+        const uint64_t ad = addr_out(e, ok);
+        if (ok) a.out[ad] = v;
+      }
+    }
+    __syncthreads();  // LDS is reused by the next tile
+  }
+}
+
+// -------------------------------------------------------------------------
+// strided<h>
+
+template <int LH, bool INV, bool DIT, int MODE>
+__global__ __launch_bounds__(NT) void ntt_strided_kernel(FastArgs a) {
+  constexpr int LT = 12 - LH;         // log2 sub-DFTs (consecutive r) per tile
+  constexpr int T = 1 << LT;
+  constexpr int NL = LH > 4 ? 2 : 1;
+  constexpr int A1 = NL == 2 ? LH - 4 : 4;  // second level width
+  constexpr int G1 = 16 >> A1;
+  constexpr int LS0 = LH - 4;               // level-0 stride (log2)
+  __shared__ uint64_t lds[TILE];
+  const uint32_t t = threadIdx.x;
+  const uint32_t log_r = a.log_N - LH;
+  const uint32_t r0 = blockIdx.x << LT;
+
+  // level-0 mapping: q = t % T, rp = t / T, j = i * 2^LS0 + rp
+  const uint32_t q0 = t & (T - 1u), rp0 = t >> LT;
+  // last-level mapping (level 1, or level 0 when h = 4): q = gi % T, b = gi / T, j = b * 2^A1 + i
+  auto lastj = [&](int g, int i) -> uint32_t {
+    if constexpr (NL == 2) return (((t + NT * (uint32_t)g) >> LT) << A1) + (uint32_t)i;
+    else return ((uint32_t)i << LS0) + rp0;
+  };
+  auto lastq = [&](int g) -> uint32_t {
+    if constexpr (NL == 2) return (t + NT * (uint32_t)g) & (T - 1u);
+    else return q0;
+  };
+
+  uint64_t tw0[15];
+  if constexpr (NL == 2) load_level_twists<INV>(a.tab, rp0, LH, tw0);
+
+  // pass twist omega_N^((r0 + q) * brev_h(j)) in the last-level mapping, with
+  // n^-1 folded in when this pass carries the inverse scaling
+  uint64_t pt[16];
+#pragma unroll
+  for (int g = 0; g < G1; ++g)
+#pragma unroll
+    for (int i = 0; i < (1 << A1); ++i) {
+      const uint32_t j = lastj(g, i);
+      uint32_t e = ((r0 + lastq(g)) * brev_bits(j, LH)) << (32u - a.log_N);
+      if (INV) e = 0u - e;
+      uint64_t w = pow_w(a.tab, e);
+      if constexpr (MODE == MODE_NINV) w = gl::mul(w, a.n_inv);
+      pt[g * (1 << A1) + i] = w;
+    }
+  // coset factors at column positions j * R + r0 + q (this is the top pass:
+  // one block per column), level-0 mapping
+  uint64_t cf[MODE == MODE_LDE_IN || MODE == MODE_LDE_OUT ? 16 : 1];
+  if constexpr (MODE == MODE_LDE_IN || MODE == MODE_LDE_OUT) {
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      const uint32_t pos = ((((uint32_t)i << LS0) + rp0) << log_r) + r0 + q0;
+      if constexpr (MODE == MODE_LDE_IN) {
+        cf[i] = coset_pow(a.tab, a.coset_exp, pos, false);  // n2b input: natural index
+      } else {
+        cf[i] = gl::mul(a.n_inv, coset_pow(a.tab, a.coset_exp, pos, true));  // b2n output: natural index
+      }
+    }
+  }
+
+  const uint32_t blocks_per_col_log = a.log_n - a.log_N;
+  const uint64_t total = (uint64_t)a.num_cols << blocks_per_col_log;
+  const uint64_t p_begin = (uint64_t)blockIdx.y * a.blocks_per_wg;
+  const uint64_t p_end = p_begin + a.blocks_per_wg < total ? p_begin + a.blocks_per_wg : total;
+  for (uint64_t p = p_begin; p < p_end; ++p) {
+    const uint64_t col = p >> blocks_per_col_log;
+    const uint64_t blk = p & ((1ull << blocks_per_col_log) - 1ull);
+    const uint64_t boff = (blk << a.log_N) + r0;
+    const uint64_t *in = a.in + col * a.stride_in + boff;
+    uint64_t *out = a.out + col * a.stride_out + boff;
+    uint64_t x[16];
+    if constexpr (!DIT) {
+#pragma unroll
+      for (int i = 0; i < 16; ++i) x[i] = in[((uint64_t)(((uint32_t)i << LS0) + rp0) << log_r) + q0];
+      if constexpr (MODE == MODE_LDE_IN) {
+#pragma unroll
+        for (int i = 0; i < 16; ++i) x[i] = gl::mul(x[i], cf[i]);
+      }
+      dft_dif<4, INV>(x);
+      if constexpr (NL == 2) {
+        twist_mul<16>(x, tw0);
+        // LDS row-major by j: element (q, j) at j * T + q
+#pragma unroll
+        for (int i = 0; i < 16; ++i) lds[((((uint32_t)i << LS0) + rp0) << LT) + q0] = x[i];
+        __syncthreads();
+#pragma unroll
+        for (int g = 0; g < G1; ++g)
+#pragma unroll
+          for (int i = 0; i < (1 << A1); ++i) x[g * (1 << A1) + i] = lds[(lastj(g, i) << LT) + lastq(g)];
+        __syncthreads();  // the next iteration rewrites LDS
+#pragma unroll
+        for (int g = 0; g < G1; ++g) dft_dif<A1, INV>(x + g * (1 << A1));
+      }
+#pragma unroll
+      for (int g = 0; g < G1; ++g)
+#pragma unroll
+        for (int i = 0; i < (1 << A1); ++i) {
+          const int k = g * (1 << A1) + i;
+          out[((uint64_t)lastj(g, i) << log_r) + lastq(g)] = gl::mul(x[k], pt[k]);
+        }
+    } else {
+#pragma unroll
+      for (int g = 0; g < G1; ++g)
+#pragma unroll
+        for (int i = 0; i < (1 << A1); ++i) {
+          const int k = g * (1 << A1) + i;
+          x[k] = gl::mul(in[((uint64_t)lastj(g, i) << log_r) + lastq(g)], pt[k]);
+        }
+      if constexpr (NL == 2) {
+#pragma unroll
+        for (int g = 0; g < G1; ++g) dft_dit<A1, INV>(x + g * (1 << A1));
+#pragma unroll
+        for (int g = 0; g < G1; ++g)
+#pragma unroll
+          for (int i = 0; i < (1 << A1); ++i) lds[(lastj(g, i) << LT) + lastq(g)] = x[g * (1 << A1) + i];
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < 16; ++i) x[i] = lds[((((uint32_t)i << LS0) + rp0) << LT) + q0];
+        __syncthreads();
+        twist_mul<16>(x, tw0);
+      }
+      dft_dit<4, INV>(x);
+#pragma unroll
+      for (int i = 0; i < 16; ++i) {
+        uint64_t v = x[i];
+        if constexpr (MODE == MODE_LDE_OUT) v = gl::mul(v, cf[i]);
+        if (a.last) v = gl::canon(v);
+        out[((uint64_t)(((uint32_t)i << LS0) + rp0) << log_r) + q0] = v;
+      }
+    }
+  }
+}
+
+// -------------------------------------------------------------------------
+// launch dispatch (only the (direction, mode) pairs the planner can ask for)
+
+template <int LH, bool INV, bool DIT>
+hipError_t contig_mode(const FastArgs &a, int mode, dim3 grid, hipStream_t s) {
+  switch (mode) {
+  case MODE_NONE: ntt_contig_kernel<LH, INV, DIT, MODE_NONE><<<grid, NT, 0, s>>>(a); break;
+  case MODE_NINV: ntt_contig_kernel<LH, INV, DIT, MODE_NINV><<<grid, NT, 0, s>>>(a); break;
+  case MODE_LDE_IN: ntt_contig_kernel<LH, INV, DIT, MODE_LDE_IN><<<grid, NT, 0, s>>>(a); break;
+  case MODE_LDE_OUT: ntt_contig_kernel<LH, INV, DIT, MODE_LDE_OUT><<<grid, NT, 0, s>>>(a); break;
+  default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+template <int LH>
+hipError_t contig_dir(const FastArgs &a, bool inv, bool dit, int mode, dim3 grid, hipStream_t s) {
+  // forward transforms never scale outputs; inverse ones never scale inputs
+  if (inv ? mode == MODE_LDE_IN : (mode == MODE_NINV || mode == MODE_LDE_OUT)) return hipErrorInvalidValue;
+  if (inv) return dit ? contig_mode<LH, true, true>(a, mode, grid, s) : contig_mode<LH, true, false>(a, mode, grid, s);
+  return dit ? contig_mode<LH, false, true>(a, mode, grid, s) : contig_mode<LH, false, false>(a, mode, grid, s);
+}
+
+template <int LH, bool INV, bool DIT>
+hipError_t strided_mode(const FastArgs &a, int mode, dim3 grid, hipStream_t s) {
+  switch (mode) {
+  case MODE_NONE: ntt_strided_kernel<LH, INV, DIT, MODE_NONE><<<grid, NT, 0, s>>>(a); break;
+  case MODE_NINV:
+    if constexpr (INV) { ntt_strided_kernel<LH, INV, DIT, MODE_NINV><<<grid, NT, 0, s>>>(a); break; }
+    return hipErrorInvalidValue;
+  case MODE_LDE_IN:
+    if constexpr (!INV && !DIT) { ntt_strided_kernel<LH, INV, DIT, MODE_LDE_IN><<<grid, NT, 0, s>>>(a); break; }
+    return hipErrorInvalidValue;
+  case MODE_LDE_OUT:
+    if constexpr (INV && DIT) { ntt_strided_kernel<LH, INV, DIT, MODE_LDE_OUT><<<grid, NT, 0, s>>>(a); break; }
+    return hipErrorInvalidValue;
+  default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+template <int LH>
+hipError_t strided_dir(const FastArgs &a, bool inv, bool dit, int mode, dim3 grid, hipStream_t s) {
+  if (inv) return dit ? strided_mode<LH, true, true>(a, mode, grid, s) : strided_mode<LH, true, false>(a, mode, grid, s);
+  return dit ? strided_mode<LH, false, true>(a, mode, grid, s) : strided_mode<LH, false, false>(a, mode, grid, s);
+}
+
+}  // namespace
+
+hipError_t launch_contig(const FastArgs &a, int log_h, bool inverse, bool dit, int mode, hipStream_t stream) {
+  if (a.tiles == 0) return hipSuccess;
+  const uint32_t max_wgs = 2048;
+  const dim3 grid(a.tiles < max_wgs ? a.tiles : max_wgs);
+  switch (log_h) {
+  case 8: return contig_dir<8>(a, inverse, dit, mode, grid, stream);
+  case 9: return contig_dir<9>(a, inverse, dit, mode, grid, stream);
+  case 10: return contig_dir<10>(a, inverse, dit, mode, grid, stream);
+  case 11: return contig_dir<11>(a, inverse, dit, mode, grid, stream);
+  case 12: return contig_dir<12>(a, inverse, dit, mode, grid, stream);
+  default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_strided(const FastArgs &a, int log_h, bool inverse, bool dit, int mode, uint32_t positions,
+                          uint32_t groups, hipStream_t stream) {
+  if (positions == 0 || groups == 0) return hipSuccess;
+  if (groups > 65535) return hipErrorInvalidConfiguration;
+  const dim3 grid(positions, groups);
+  switch (log_h) {
+  case 4: return strided_dir<4>(a, inverse, dit, mode, grid, stream);
+  case 5: return strided_dir<5>(a, inverse, dit, mode, grid, stream);
+  case 6: return strided_dir<6>(a, inverse, dit, mode, grid, stream);
+  case 7: return strided_dir<7>(a, inverse, dit, mode, grid, stream);
+  case 8: return strided_dir<8>(a, inverse, dit, mode, grid, stream);
+  default: return hipErrorInvalidValue;
+  }
+}
+
+}  // namespace bc

@@ -47,6 +47,36 @@ struct PassArgs {
   uint32_t coset_exp;  // coset_index << (32 - log_n - log_ext)
 };
 
+// Arguments of the register-resident pass kernels (ntt_fast.hip).
+struct FastArgs {
+  const uint64_t *in;
+  uint64_t *out;
+  uint64_t stride_in;
+  uint64_t stride_out;
+  uint64_t n_inv;
+  Tables tab;
+  uint32_t num_cols;
+  uint32_t log_n;
+  uint32_t log_N;          // strided passes: block length this pass works inside
+  uint32_t log_ext;
+  uint32_t coset_exp;
+  uint32_t last;           // pass writes the final (canonical) output
+  uint32_t tiles;          // contiguous passes: number of 4096-point tiles
+  uint32_t blocks_per_wg;  // strided passes: (column, block) pairs per workgroup
+};
+
+// Scaling folded into a pass (see plan in ntt_api.hip).
+enum PassMode : int {
+  MODE_NONE = 0,     // nothing beyond the transform
+  MODE_NINV = 1,     // multiply by n^-1
+  MODE_LDE_IN = 2,   // multiply inputs by s^i (forward coset transform)
+  MODE_LDE_OUT = 3,  // multiply outputs by n^-1 s^-i (inverse coset transform)
+};
+
+hipError_t launch_contig(const FastArgs &a, int log_h, bool inverse, bool dit, int mode, hipStream_t stream);
+hipError_t launch_strided(const FastArgs &a, int log_h, bool inverse, bool dit, int mode, uint32_t positions,
+                          uint32_t groups, hipStream_t stream);
+
 constexpr uint32_t TILE_LOG = 12;
 constexpr uint32_t TILE = 1u << TILE_LOG;
 constexpr uint32_t TILE_THREADS = 256;

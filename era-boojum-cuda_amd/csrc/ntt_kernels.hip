@@ -27,88 +27,11 @@
 // powers of omega_16, which are powers of two mod p, so they cost shifts, not
 // multiplications.  Between levels the same twist rule applies with N -> M
 // (the level's block) and comes from the coarse omega table.
-#include "gl_field.hpp"
-#include "ntt_internal.hpp"
-
-#include <utility>
+#include "ntt_device.hpp"
 
 namespace bc {
 
 namespace {
-
-template <class F, int... I>
-__device__ __forceinline__ void static_for_impl(F &&f, std::integer_sequence<int, I...>) {
-  (f(std::integral_constant<int, I>{}), ...);
-}
-template <int N, class F> __device__ __forceinline__ void static_for(F &&f) {
-  static_for_impl(f, std::make_integer_sequence<int, N>{});
-}
-
-__device__ __forceinline__ uint32_t brev_bits(uint32_t x, uint32_t bits) {
-  return bits ? (__brev(x) >> (32u - bits)) : 0u;
-}
-
-// omega_{2^32}^E.
-__device__ __forceinline__ uint64_t pow_w(const Tables &t, uint32_t e) {
-  const uint64_t c = t.w_coarse[e >> TABLE_LOG];
-  const uint32_t f = e & (TABLE_LEN - 1);
-  return f ? gl::mul(c, t.w_fine[f]) : c;
-}
-
-// omega_{2^32}^E for E a multiple of 2^16.
-__device__ __forceinline__ uint64_t pow_w_coarse(const Tables &t, uint32_t e) { return t.w_coarse[e >> TABLE_LOG]; }
-
-__device__ __forceinline__ uint64_t pow_g(const Tables &t, uint32_t i, bool inverse) {
-  const uint64_t *fine = inverse ? t.gi_fine : t.g_fine;
-  const uint64_t *coarse = inverse ? t.gi_coarse : t.g_coarse;
-  const uint64_t c = coarse[i >> TABLE_LOG];
-  const uint32_t f = i & (TABLE_LEN - 1);
-  return f ? gl::mul(c, fine[f]) : c;
-}
-
-// s^i (or s^-i) for the coset shift s = g * omega_{n 2^ext}^coset.
-__device__ __forceinline__ uint64_t coset_power(const PassArgs &a, uint32_t i, bool inverse) {
-  uint64_t v = pow_g(a.tab, i, inverse);
-  if (a.coset_exp) {
-    uint32_t e = i * a.coset_exp;  // (coset * i) << (32 - log_n - log_ext), mod 2^32
-    if (inverse) e = 0u - e;
-    v = gl::mul(v, pow_w(a.tab, e));
-  }
-  return v;
-}
-
-// In-register DFT of 2^A points, natural input -> bit-reversed output.
-template <int A, bool INV> __device__ __forceinline__ void dft_dif(uint64_t *x) {
-  static_for<A>([&](auto s) {
-    constexpr int LH = A - 1 - decltype(s)::value;
-    constexpr int HALF = 1 << LH;
-    static_for<(1 << A) / 2>([&](auto k) {
-      constexpr int K = decltype(k)::value;
-      constexpr int T = K % HALF;
-      constexpr int I0 = (K / HALF) * 2 * HALF + T;
-      const uint64_t u = x[I0], v = x[I0 + HALF];
-      x[I0] = gl::add(u, v);
-      x[I0 + HALF] = gl::mul_pow2<gl::root_pow2_exp(LH + 1, T, INV)>(gl::sub(u, v));
-    });
-  });
-}
-
-// In-register DFT of 2^A points, bit-reversed input -> natural output.
-template <int A, bool INV> __device__ __forceinline__ void dft_dit(uint64_t *x) {
-  static_for<A>([&](auto s) {
-    constexpr int LH = decltype(s)::value;
-    constexpr int HALF = 1 << LH;
-    static_for<(1 << A) / 2>([&](auto k) {
-      constexpr int K = decltype(k)::value;
-      constexpr int T = K % HALF;
-      constexpr int I0 = (K / HALF) * 2 * HALF + T;
-      const uint64_t u = x[I0];
-      const uint64_t v = gl::mul_pow2<gl::root_pow2_exp(LH + 1, T, INV)>(x[I0 + HALF]);
-      x[I0] = gl::add(u, v);
-      x[I0 + HALF] = gl::sub(u, v);
-    });
-  });
-}
 
 // LDS position of point j of sub-transform q.  The XOR only permutes the low
 // four bits of j (by a function of q and j >> 4), which spreads both the
@@ -229,7 +152,7 @@ __global__ __launch_bounds__(TILE_THREADS) void ntt_pass_kernel(PassArgs a) {
       if (!INV && a.first && a.log_ext) {
         // input position pos holds x_i with i = pos (n2b) or brev(pos) (b2n)
         const uint32_t idx = DIT ? brev_bits((uint32_t)el.pos, a.log_n) : (uint32_t)el.pos;
-        v = gl::mul(v, coset_power(a, idx, false));
+        v = gl::mul(v, coset_pow(a.tab, a.coset_exp, idx, false));
       }
       if (DIT && log_r > 0) {
         uint32_t ex = (el.r * brev_bits(el.j, log_h)) << (32u - a.log_N);
@@ -280,7 +203,7 @@ __global__ __launch_bounds__(TILE_THREADS) void ntt_pass_kernel(PassArgs a) {
         if (a.log_ext) {
           // output position pos holds y_i with i = brev(pos) (n2b) or pos (b2n)
           const uint32_t idx = DIT ? (uint32_t)el.pos : brev_bits((uint32_t)el.pos, a.log_n);
-          v = gl::mul(v, coset_power(a, idx, true));
+          v = gl::mul(v, coset_pow(a.tab, a.coset_exp, idx, true));
         }
       }
       v = gl::canon(v);
