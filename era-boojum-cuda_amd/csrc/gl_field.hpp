@@ -1,9 +1,12 @@
 // Goldilocks field arithmetic, p = 2^64 - 2^32 + 1, for host and gfx950.
 //
-// Semantics follow the reference's base field (native/goldilocks.cuh:90-185):
-// elements are u64 words that may hold any representative in [0, 2^64);
-// add/sub/mul return a representative congruent mod p, and canon() maps to
-// [0, p).  The implementation is derived from the two identities
+// Semantics follow the reference's base field (native/goldilocks.cuh:90-185),
+// with one difference in representation: the reference lets any u64 stand for
+// its residue, while here every value inside a transform is CANONICAL, in
+// [0, p).  Inputs are canonicalized once when loaded; add/sub require and
+// return canonical values (which removes the double-wrap corrections a
+// non-canonical add needs), mul and mul_pow2 accept any u64 and return
+// canonical values.  The implementation is derived from the two identities
 //   2^64 = 2^32 - 1 (mod p)      2^96 = -1 (mod p)
 // and is written for AMD's 32-bit VALU: 64x64 products are four
 // v_mad_u64_u32 steps, carries are read back from the add/sub carry-out.
@@ -22,96 +25,138 @@ namespace gl {
 constexpr uint64_t P = 0xFFFFFFFF00000001ull;
 constexpr uint64_t EPS = 0x00000000FFFFFFFFull;  // 2^64 mod p
 
-GL_FN uint64_t canon(uint64_t x) { return x >= P ? x - P : x; }
-
-// a + b mod p for any u64 inputs.
-GL_FN uint64_t add(uint64_t a, uint64_t b) {
-  uint64_t s;
-  bool c1 = __builtin_add_overflow(a, b, &s);
-  // 2^64 wraps to EPS; a second wrap is possible only for non-canonical
-  // inputs and then leaves s < EPS, so the third add cannot wrap.
-  bool c2 = __builtin_add_overflow(s, c1 ? EPS : 0ull, &s);
-  return s + (c2 ? EPS : 0ull);
+// Canonical representative in [0, p) of any u64.  x >= p exactly when
+// x + EPS wraps, so no comparison with p is needed.
+GL_FN uint64_t canon(uint64_t x) {
+  const uint64_t t = x + EPS;
+  return t < x ? t : x;
 }
 
-// a - b mod p for any u64 inputs.
+// a + b mod p for CANONICAL a, b; result canonical.  a + b < 2p, so either the
+// 64-bit add wraps (result a + b - 2^64 + EPS, no second wrap) or the sum is
+// below 2^64 and at most one p comes off (detected by s + EPS wrapping).
+GL_FN uint64_t add(uint64_t a, uint64_t b) {
+  const uint64_t s = a + b, t = s + EPS;
+  return ((s < a) | (t < s)) ? t : s;
+}
+
+// a - b mod p for CANONICAL a, b; result canonical.
 GL_FN uint64_t sub(uint64_t a, uint64_t b) {
-  uint64_t d;
-  bool b1 = __builtin_sub_overflow(a, b, &d);
-  bool b2 = __builtin_sub_overflow(d, b1 ? EPS : 0ull, &d);
-  return d - (b2 ? EPS : 0ull);
+  const uint64_t d = a - b;
+  return a < b ? d + P : d;
 }
 
 GL_FN uint64_t neg(uint64_t a) { return sub(0ull, a); }
 
-// Reduce lo + hi * 2^64.  hi = h1 * 2^32 + h0 contributes h0 * (2^32 - 1) - h1.
-GL_FN uint64_t reduce128(uint64_t lo, uint64_t hi) {
-  const uint64_t h0 = hi & 0xFFFFFFFFull;
-  const uint64_t h1 = hi >> 32;
-  uint64_t t;
-  // lo - h1: on borrow the wrapped value is >= 2^64 - 2^32, so one EPS
-  // correction cannot borrow again.
-  bool b = __builtin_sub_overflow(lo, h1, &t);
-  t -= b ? EPS : 0ull;
-  const uint64_t u = (h0 << 32) - h0;  // h0 * (2^32 - 1) < 2^64
-  uint64_t r;
-  bool c = __builtin_add_overflow(t, u, &r);
-  // t + u - 2^64 + EPS < u + EPS < 2^64: no second wrap.
-  return r + (c ? EPS : 0ull);
+// Canonical value of x + d * 2^64 for d in {-1, 0, +1} (minus / plus flags),
+// given bounds that make x + EPS (plus) and x + P (minus) canonical.
+GL_FN uint64_t fold(uint64_t x, bool plus, bool minus) {
+  const bool add_eps = plus | (!minus & (x >= P));
+  const uint32_t lo = add_eps ? 0xFFFFFFFFu : (minus ? 1u : 0u);
+  const uint32_t hi = minus ? 0xFFFFFFFFu : 0u;
+  return x + (((uint64_t)hi << 32) | lo);
 }
 
-// Full 64x64 -> 128 product built from four 32x32+64 multiply-adds.
-GL_FN void mul_wide(uint64_t a, uint64_t b, uint64_t &lo, uint64_t &hi) {
-  const uint64_t a0 = a & 0xFFFFFFFFull, a1 = a >> 32;
-  const uint64_t b0 = b & 0xFFFFFFFFull, b1 = b >> 32;
-  const uint64_t p00 = a0 * b0;
-  const uint64_t t1 = a0 * b1 + (p00 >> 32);          // < 2^64
-  const uint64_t t2 = a1 * b0 + (t1 & 0xFFFFFFFFull);  // < 2^64
-  lo = (p00 & 0xFFFFFFFFull) | (t2 << 32);
-  hi = a1 * b1 + (t1 >> 32) + (t2 >> 32);               // <= 2^64 - 1
+// Reduce l0 + l1 2^32 + l2 2^64 + l3 2^96 (32-bit limbs) to canonical:
+// 2^64 = 2^32 - 1 and 2^96 = -1 give (l1 + l2) 2^32 + l0 - (l2 + l3), a value
+// in (-2^33, 2^65) that one fold corrects.
+GL_FN uint64_t reduce_limbs(uint32_t l0, uint32_t l1, uint32_t l2, uint32_t l3) {
+  unsigned c, sc, b0, b1;
+  uint32_t hi = __builtin_addc(l1, l2, 0u, &c);
+  const uint32_t ssum = __builtin_addc(l2, l3, 0u, &sc);
+  const uint32_t lo = __builtin_subc(l0, ssum, 0u, &b0);
+  hi = __builtin_subc(hi, sc, b0, &b1);
+  // c set, b1 clear: (hi:lo) <= 2^64 - 2^33, so + EPS stays below p;
+  // b1 set, c clear: the value is above -2^33, so + p is canonical.
+  return fold(((uint64_t)hi << 32) | lo, c && !b1, b1 && !c);
 }
 
+// a * b mod p for any u64 a, b; result canonical.  The 128-bit product takes
+// four 32x32+64 multiply-adds: the middle column's addend (p00_hi, p11_lo)
+// already has the right weight, and its carries land in the top limb.
 GL_FN uint64_t mul(uint64_t a, uint64_t b) {
-  uint64_t lo, hi;
-  mul_wide(a, b, lo, hi);
-  return reduce128(lo, hi);
+  const uint32_t a0 = (uint32_t)a, a1 = (uint32_t)(a >> 32);
+  const uint32_t b0 = (uint32_t)b, b1 = (uint32_t)(b >> 32);
+  const uint64_t p00 = (uint64_t)a0 * b0;
+  const uint64_t p11 = (uint64_t)a1 * b1;
+  uint64_t m;
+  const bool c1 = __builtin_add_overflow((uint64_t)a0 * b1, (p11 << 32) | (p00 >> 32), &m);
+  const bool c2 = __builtin_add_overflow((uint64_t)a1 * b0, m, &m);
+  const uint32_t l3 = (uint32_t)(p11 >> 32) + (uint32_t)c1 + (uint32_t)c2;  // exact: product < 2^128
+  return reduce_limbs((uint32_t)p00, (uint32_t)m, (uint32_t)(m >> 32), l3);
 }
 
-// Reduce lo + hi * 2^64 for hi < 2^32: hi * 2^64 = hi * (2^32 - 1).
-GL_FN uint64_t reduce96(uint64_t lo, uint32_t hi) {
-  const uint64_t u = ((uint64_t)hi << 32) - hi;
-  uint64_t r;
-  bool c = __builtin_add_overflow(lo, u, &r);
-  // lo + u - 2^64 + EPS < u + EPS < 2^64: no second wrap.
-  return r + (c ? EPS : 0ull);
-}
-
-// x * 2^E mod p for a compile-time exponent 0 <= E < 96.  (2 has order 192
-// and 2^96 = -1; callers fold the sign of larger exponents into the
-// surrounding add/sub.)
-template <unsigned E> GL_FN uint64_t mul_pow2(uint64_t x) {
-  static_assert(E < 96, "use the sign-folded form for exponents >= 96");
-  if constexpr (E == 0) {
-    return x;
-  } else if constexpr (E <= 32) {
-    return reduce96(x << E, (uint32_t)(x >> (64 - E)));
-  } else if constexpr (E < 64) {
-    return reduce128(x << E, x >> (64 - E));
-  } else if constexpr (E == 64) {
-    return reduce128(0ull, x);
+// x * 2^E for 0 < E < 64, canonical result, any u64 x.
+template <unsigned E> GL_FN uint64_t pow2_direct(uint64_t x) {
+  static_assert(E > 0 && E < 64, "");
+  const uint32_t x0 = (uint32_t)x, x1 = (uint32_t)(x >> 32);
+  if constexpr (E < 32) {
+    // x << E = y0 + y1 2^32 + y2 2^64 = (y1 + y2) 2^32 + y0 - y2, y2 < 2^E
+    const uint32_t y0 = x0 << E;
+    const uint32_t y1 = (x1 << E) | (x0 >> (32 - E));
+    const uint32_t y2 = x1 >> (32 - E);
+    unsigned c, b0, b1;
+    uint32_t hi = __builtin_addc(y1, y2, 0u, &c);
+    const uint32_t lo = __builtin_subc(y0, y2, 0u, &b0);
+    hi = __builtin_subc(hi, 0u, b0, &b1);
+    // value = c 2^64 + (hi:lo) - b1 2^64; with c set, hi < y2 < 2^E keeps
+    // (hi:lo) + EPS below p; c = 0, b1 = 1 cannot happen
+    return fold(((uint64_t)hi << 32) | lo, c && !b1, false);
+  } else if constexpr (E == 32) {
+    return reduce_limbs(0u, x0, x1, 0u);  // x 2^32 = x0 2^32 + x1 2^64
   } else {
-    // E in (64, 96): x * 2^(E-64) = lo + hi * 2^64 with hi < 2^32, then
-    // times 2^64: lo * 2^64 + hi * 2^128 = lo * 2^64 - hi * 2^32.
-    const uint64_t lo = x << (E - 64);
-    const uint64_t hi = x >> (128 - E);
-    return sub(reduce128(0ull, lo), hi << 32);
+    // 2^E = -2^-K with K = 96 - E in (32, 64): split x = q 2^K + r, then
+    // x 2^E = r 2^E - q with r 2^E < 2^96:  r 2^E = (Lhi 2^32) + H 2^64,
+    // H 2^64 = H 2^32 - H, so x 2^E = (Lhi + H) 2^32 - (H + q).
+    constexpr unsigned K = 96 - E;
+    const uint32_t lhi = x0 << (E - 32);
+    const uint32_t rhi = x1 & ((1u << (K - 32)) - 1u);
+    const uint32_t h = (uint32_t)((((uint64_t)rhi << 32) | x0) >> (K - 32));
+    const uint32_t q = x1 >> (K - 32);
+    unsigned c, sc, b0, b1;
+    uint32_t hi = __builtin_addc(lhi, h, 0u, &c);
+    const uint32_t ssum = __builtin_addc(h, q, 0u, &sc);
+    const uint32_t lo = __builtin_subc(0u, ssum, 0u, &b0);
+    hi = __builtin_subc(hi, sc, b0, &b1);
+    return fold(((uint64_t)hi << 32) | lo, c && !b1, b1 && !c);
   }
 }
 
-// x * 2^E for any 0 <= E < 192.
-template <unsigned E> GL_FN uint64_t mul_pow2_any(uint64_t x) {
-  if constexpr (E >= 96) return neg(mul_pow2<E - 96>(x));
-  else return mul_pow2<E>(x);
+// x * 2^-K for 0 < K <= 32, canonical result, any u64 x:
+// x = q 2^K + r, r 2^-K = -r 2^(96-K) = -(r << (32-K)) (2^32 - 1), so
+// x 2^-K = q + r' - r' 2^32 with r' = r << (32 - K) < 2^32.
+template <unsigned K> GL_FN uint64_t pow2_inverse(uint64_t x) {
+  static_assert(K > 0 && K <= 32, "");
+  const uint32_t rp = K == 32 ? (uint32_t)x : ((uint32_t)x << ((32 - K) % 32));
+  const uint64_t a = (x >> K) + rp;  // < 2^(64-K) + 2^32, no wrap
+  const uint32_t ahi = (uint32_t)(a >> 32);
+  unsigned b;
+  const uint32_t hi = __builtin_subc(ahi, rp, 0u, &b);
+  const uint64_t v = ((uint64_t)hi << 32) | (uint32_t)a;
+  // no borrow: v < 2^(64-K) + 2^32 < p; borrow: v - 2^64 in (-2^64, 0), + p
+  return b ? v + P : v;
+}
+
+// x * 2^e mod p for a compile-time 0 <= e < 192, as sign * value with value
+// canonical (2^96 = -1 and 2^e = 2^(e-192)); callers fold the sign into the
+// surrounding add/sub.  e = 0 passes x through and expects a canonical x.
+template <unsigned e> struct Pow2 {
+  static constexpr unsigned E = e % 192u;
+  static constexpr bool NEG = (E >= 64 && E < 96) || (E >= 96 && E < 160);
+  GL_FN static uint64_t value(uint64_t x) {
+    if constexpr (E == 0) return x;
+    else if constexpr (E < 64) return pow2_direct<E>(x);
+    else if constexpr (E < 96) return pow2_inverse<96 - E>(x);
+    else if constexpr (E == 96) return x;  // -x, sign in NEG
+    else if constexpr (E < 160) return pow2_direct<E - 96>(x);
+    else return pow2_inverse<192 - E>(x);
+  }
+};
+
+// x * 2^E mod p, canonical, for 0 <= E < 192 (canonical x when E % 96 == 0).
+template <unsigned E> GL_FN uint64_t mul_pow2(uint64_t x) {
+  const uint64_t v = Pow2<E>::value(x);
+  return Pow2<E>::NEG ? neg(v) : v;
 }
 
 GL_FN uint64_t pow(uint64_t b, uint64_t e) {

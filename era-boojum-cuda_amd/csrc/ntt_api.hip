@@ -109,7 +109,7 @@ extern "C" int bc_context_create(uint32_t powers_of_w_coarse_log_count, uint32_t
   if (err != hipSuccess) return err;
   if (dev < 0 || dev >= MAX_DEVICES) return BC_ERROR_INVALID_VALUE;
 
-  std::vector<uint64_t> host((size_t)NUM_TABLES * TABLE_LEN);
+  std::vector<uint64_t> host((size_t)NUM_TABLES * TABLE_LEN + 2 * LVL_TW_BLOCK);
   const uint64_t w = gl::OMEGA_2_32;
   const uint64_t w_coarse = gl::pow(w, TABLE_LEN);
   const uint64_t g = gl::GENERATOR;
@@ -120,6 +120,20 @@ extern "C" int bc_context_create(uint32_t powers_of_w_coarse_log_count, uint32_t
   fill_powers(&host[3 * TABLE_LEN], TABLE_LEN, gl::pow(g, TABLE_LEN));
   fill_powers(&host[4 * TABLE_LEN], TABLE_LEN, gi);
   fill_powers(&host[5 * TABLE_LEN], TABLE_LEN, gl::pow(gi, TABLE_LEN));
+  // radix-16 level twists: omega_M^(+-rp * brev4(i)), omega_M = omega_{2^32}^(2^(32-m))
+  uint64_t *lvl = &host[(size_t)NUM_TABLES * TABLE_LEN];
+  for (uint32_t m = LVL_TW_MIN_LOG; m <= LVL_TW_MAX_LOG; ++m) {
+    const uint32_t S = 1u << (m - 4);
+    const uint64_t wm = gl::pow(w, 1ull << (32 - m));
+    const uint64_t wm_inv = gl::inv(wm);
+    for (uint32_t i = 1; i < 16; ++i) {
+      const uint32_t bi = ((i & 1) << 3) | ((i & 2) << 1) | ((i & 4) >> 1) | ((i & 8) >> 3);
+      for (uint32_t rp = 0; rp < S; ++rp) {
+        lvl[lvl_tw_offset(m) + (i - 1) * S + rp] = gl::pow(wm, (uint64_t)rp * bi);
+        lvl[LVL_TW_BLOCK + lvl_tw_offset(m) + (i - 1) * S + rp] = gl::pow(wm_inv, (uint64_t)rp * bi);
+      }
+    }
+  }
 
   bc_context *ctx = new bc_context();
   ctx->device = dev;
@@ -134,6 +148,7 @@ extern "C" int bc_context_create(uint32_t powers_of_w_coarse_log_count, uint32_t
     delete ctx;
     return err;
   }
+  ctx->tab.lvl_tw = ctx->tables + (size_t)NUM_TABLES * TABLE_LEN;
   ctx->tab.w_fine = ctx->tables + 0 * TABLE_LEN;
   ctx->tab.w_coarse = ctx->tables + 1 * TABLE_LEN;
   ctx->tab.g_fine = ctx->tables + 2 * TABLE_LEN;
@@ -202,6 +217,7 @@ extern "C" int bc_batch_ntt_internal(const uint64_t *inputs, uint64_t *outputs, 
     a.stride_in = first ? stride_in : stride_out;
     a.out = outputs;
     a.stride_out = stride_out;
+    a.first = first;
     a.last = last;
     hipError_t err;
     if (p == np - 1) {

@@ -39,8 +39,8 @@ __device__ __forceinline__ uint64_t pow_g(const Tables &t, uint32_t i, bool inve
 }
 
 // In-register DFT of 2^A points, natural input -> bit-reversed output
-// (decimation in frequency).  Twiddles omega_{2^k}^t are +-2^e with e < 96;
-// a minus sign is folded into the subtraction.
+// (decimation in frequency).  Twiddles omega_{2^k}^t are powers of two,
+// applied as sign * canonical value with the sign folded into the add/sub.
 template <int A, bool INV> __device__ __forceinline__ void dft_dif(uint64_t *x) {
   static_for<A>([&](auto s) {
     constexpr int LH = A - 1 - decltype(s)::value;
@@ -49,11 +49,10 @@ template <int A, bool INV> __device__ __forceinline__ void dft_dif(uint64_t *x) 
       constexpr int K = decltype(k)::value;
       constexpr int T = K % HALF;
       constexpr int I0 = (K / HALF) * 2 * HALF + T;
-      constexpr unsigned E = gl::root_pow2_exp(LH + 1, T, INV);
+      using W = gl::Pow2<gl::root_pow2_exp(LH + 1, T, INV)>;
       const uint64_t u = x[I0], v = x[I0 + HALF];
       x[I0] = gl::add(u, v);
-      if constexpr (E >= 96) x[I0 + HALF] = gl::mul_pow2<E - 96>(gl::sub(v, u));
-      else x[I0 + HALF] = gl::mul_pow2<E>(gl::sub(u, v));
+      x[I0 + HALF] = W::value(W::NEG ? gl::sub(v, u) : gl::sub(u, v));
     });
   });
 }
@@ -68,14 +67,13 @@ template <int A, bool INV> __device__ __forceinline__ void dft_dit(uint64_t *x) 
       constexpr int K = decltype(k)::value;
       constexpr int T = K % HALF;
       constexpr int I0 = (K / HALF) * 2 * HALF + T;
-      constexpr unsigned E = gl::root_pow2_exp(LH + 1, T, INV);
+      using W = gl::Pow2<gl::root_pow2_exp(LH + 1, T, INV)>;
       const uint64_t u = x[I0];
-      if constexpr (E >= 96) {
-        const uint64_t m = gl::mul_pow2<E - 96>(x[I0 + HALF]);
+      const uint64_t m = W::value(x[I0 + HALF]);
+      if constexpr (W::NEG) {
         x[I0] = gl::sub(u, m);
         x[I0 + HALF] = gl::add(u, m);
       } else {
-        const uint64_t m = gl::mul_pow2<E>(x[I0 + HALF]);
         x[I0] = gl::add(u, m);
         x[I0 + HALF] = gl::sub(u, m);
       }

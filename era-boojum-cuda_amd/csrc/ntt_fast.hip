@@ -31,20 +31,25 @@ constexpr uint32_t NT = 256;  // threads per workgroup
 __device__ __forceinline__ uint32_t pidx(uint32_t e) { return e + (e >> 4); }
 constexpr uint32_t LDS_PADDED = TILE + TILE / 16;
 
-template <int N> __device__ __forceinline__ void twist_mul(uint64_t *x, const uint64_t *tw) {
-#pragma unroll
-  for (int i = 1; i < N; ++i) x[i] = gl::mul(x[i], tw[i - 1]);
+// Level-twist table of a radix-16 level in blocks of M = 2^LM points: entry
+// (i - 1) * S + rp = omega_M^(+-rp * brev4(i)), S = M / 16.
+template <int LM, bool INV> __device__ __forceinline__ const uint64_t *lvl_table(const Tables &tab) {
+  return tab.lvl_tw + (INV ? LVL_TW_BLOCK : 0u) + lvl_tw_offset(LM);
 }
 
-// Per-thread level twist omega_M^(rp * brev4(i)), i = 1..15, M = 2^log_m.
-template <bool INV>
-__device__ __forceinline__ void load_level_twists(const Tables &tab, uint32_t rp, uint32_t log_m, uint64_t *tw) {
+// Copies a level-twist table into LDS (once per workgroup).
+template <int LM, bool INV> __device__ __forceinline__ void stage_level_twists(uint64_t *dst, const Tables &tab) {
+  constexpr uint32_t LEN = 15u << (LM - 4);
+  const uint64_t *src = lvl_table<LM, INV>(tab);
+  for (uint32_t k = threadIdx.x; k < LEN; k += NT) dst[k] = src[k];
+}
+
+// x[i] *= tw[(i - 1) * S + rp] for i = 1..15 (tw in LDS).
+template <int LM>
+__device__ __forceinline__ void level_twist_lds(uint64_t *x, const uint64_t *tw, uint32_t rp) {
+  constexpr uint32_t S = 1u << (LM - 4);
 #pragma unroll
-  for (int i = 1; i < 16; ++i) {
-    uint32_t e = (rp * (uint32_t)(__brev(i) >> 28)) << (32u - log_m);
-    if (INV) e = 0u - e;
-    tw[i - 1] = pow_w_coarse(tab, e);
-  }
+  for (int i = 1; i < 16; ++i) x[i] = gl::mul(x[i], tw[(i - 1) * S + rp]);
 }
 
 // -------------------------------------------------------------------------
@@ -73,63 +78,102 @@ __device__ __forceinline__ uint64_t contig_scale(const FastArgs &a, uint32_t pos
   }
 }
 
-template <int LH, bool INV, bool DIT, int MODE>
-__global__ __launch_bounds__(NT) void ntt_contig_kernel(FastArgs a) {
-  constexpr int NL = LH > 8 ? 3 : 2;
-  constexpr int A2 = NL == 3 ? LH - 8 : 4;  // width of the third level
-  constexpr int G2 = 16 >> A2;              // third-level groups per thread
-  constexpr int SUB = 12 - LH;              // log2 sub-DFTs per tile
-  __shared__ uint64_t lds[LDS_PADDED];
-  const uint32_t t = threadIdx.x;
-
-  uint64_t tw0[15], tw1[15];
-  load_level_twists<INV>(a.tab, t & ((1u << (LH - 4)) - 1u), LH, tw0);
-  if constexpr (NL == 3) load_level_twists<INV>(a.tab, t & ((1u << (LH - 8)) - 1u), LH - 4, tw1);
-
-  const bool multicol = a.log_n < TILE_LOG;  // tile spans 2^SUB columns (single-pass case)
-  for (uint32_t tile = blockIdx.x; tile < a.tiles; tile += gridDim.x) {
-    // Address of tile element e: in-column tiles are 4096 contiguous points.
-    uint64_t base_in, base_out;
-    uint32_t pos0 = 0, col0 = 0;
+// Addressing of a contiguous-pass tile: 4096 consecutive points of one column,
+// or (single-pass transforms shorter than 4096) 2^(12-h) whole columns.
+template <int LH> struct ContigTile {
+  uint64_t base_in, base_out;
+  uint32_t pos0, col0;
+  bool multicol;
+  __device__ __forceinline__ ContigTile(const FastArgs &a, uint32_t tile) {
+    multicol = a.log_n < TILE_LOG;
     if (!multicol) {
       const uint32_t sh = a.log_n - TILE_LOG;
       const uint64_t col = tile >> sh;
       pos0 = (tile & ((1u << sh) - 1u)) << TILE_LOG;
       base_in = col * a.stride_in + pos0;
       base_out = col * a.stride_out + pos0;
+      col0 = 0;
     } else {
-      col0 = tile << SUB;
+      col0 = tile << (12 - LH);
+      pos0 = 0;
       base_in = base_out = 0;
     }
-    auto addr_in = [&](uint32_t e, bool &ok) -> uint64_t {
-      if (!multicol) { ok = true; return base_in + e; }
-      const uint32_t c = col0 + (e >> LH);
-      ok = c < a.num_cols;
-      return (uint64_t)c * a.stride_in + (e & ((1u << LH) - 1u));
-    };
-    auto addr_out = [&](uint32_t e, bool &ok) -> uint64_t {
-      if (!multicol) { ok = true; return base_out + e; }
-      const uint32_t c = col0 + (e >> LH);
-      ok = c < a.num_cols;
-      return (uint64_t)c * a.stride_out + (e & ((1u << LH) - 1u));
-    };
-    auto pos_of = [&](uint32_t e) -> uint32_t { return multicol ? (e & ((1u << LH) - 1u)) : pos0 + e; };
+  }
+  __device__ __forceinline__ uint64_t addr(uint32_t e, uint64_t stride, uint64_t base, uint32_t ncols, bool &ok) const {
+    if (!multicol) {
+      ok = true;
+      return base + e;
+    }
+    const uint32_t c = col0 + (e >> LH);
+    ok = c < ncols;
+    return (uint64_t)c * stride + (e & ((1u << LH) - 1u));
+  }
+  __device__ __forceinline__ uint32_t pos(uint32_t e) const { return multicol ? (e & ((1u << LH) - 1u)) : pos0 + e; }
+};
 
+// Loads a tile's 16 values per thread in the level-0 mapping (lanes on
+// consecutive points).
+template <int LH>
+__device__ __forceinline__ void contig_load(const FastArgs &a, uint32_t tile, uint32_t t, uint64_t *x) {
+  const ContigTile<LH> ct(a, tile);
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    bool ok;
+    const uint64_t ad = ct.addr(celem<LH, 0, 4>(t, 0, i), a.stride_in, ct.base_in, a.num_cols, ok);
+    x[i] = ok ? a.in[ad] : 0ull;
+  }
+}
+
+template <int LH, bool INV, bool DIT, int MODE>
+__global__ __launch_bounds__(NT) void ntt_contig_kernel(FastArgs a) {
+  constexpr int NL = LH > 8 ? 3 : 2;
+  constexpr int A2 = NL == 3 ? LH - 8 : 4;  // width of the third level
+  constexpr int G2 = 16 >> A2;              // third-level groups per thread
+  constexpr int LL = NL - 1, AL = NL == 3 ? A2 : 4, GL = 16 >> AL;  // last level
+  constexpr uint32_t TW1_LEN = NL == 3 ? (15u << (LH - 8)) : 1u;
+  __shared__ uint64_t lds[LDS_PADDED];
+  __shared__ uint64_t tw1[TW1_LEN];
+  const uint32_t t = threadIdx.x;
+
+  // level-0 twists: 15 per thread, in registers for the whole kernel
+  uint64_t tw0[15];
+  {
+    const uint64_t *src = lvl_table<LH, INV>(a.tab) + (t & ((1u << (LH - 4)) - 1u));
+#pragma unroll
+    for (int i = 1; i < 16; ++i) tw0[i - 1] = src[(i - 1) << (LH - 4)];
+  }
+  // level-1 twists (at most 240 entries) in LDS
+  if constexpr (NL == 3) stage_level_twists<LH - 4, INV>(tw1, a.tab);
+  const uint32_t rp1 = t & ((1u << (LH - 8)) - 1u);
+
+  uint32_t tile = blockIdx.x;
+  uint64_t pf[16];
+  if (tile < a.tiles) contig_load<LH>(a, tile, t, pf);
+  __syncthreads();  // tw1 staged
+
+  for (; tile < a.tiles; tile += gridDim.x) {
+    const ContigTile<LH> ct(a, tile);
     uint64_t x[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) x[i] = pf[i];
+    {
+      const uint32_t next = tile + gridDim.x;
+      if (next < a.tiles) contig_load<LH>(a, next, t, pf);  // prefetch: in flight during this tile's math
+    }
+    if constexpr (MODE == MODE_LDE_IN) {
+#pragma unroll
+      for (int i = 0; i < 16; ++i) x[i] = gl::mul(x[i], contig_scale<MODE, DIT>(a, ct.pos(celem<LH, 0, 4>(t, 0, i))));
+    } else {
+      if (a.first) {
+#pragma unroll
+        for (int i = 0; i < 16; ++i) x[i] = gl::canon(x[i]);  // field ops take canonical values
+      }
+    }
+
     if constexpr (!DIT) {
-      // level 0 straight from HBM (lanes walk consecutive points)
-#pragma unroll
-      for (int i = 0; i < 16; ++i) {
-        bool ok;
-        const uint64_t ad = addr_in(celem<LH, 0, 4>(t, 0, i), ok);
-        x[i] = ok ? a.in[ad] : 0ull;
-      }
-      if constexpr (MODE == MODE_LDE_IN) {
-#pragma unroll
-        for (int i = 0; i < 16; ++i) x[i] = gl::mul(x[i], contig_scale<MODE, DIT>(a, pos_of(celem<LH, 0, 4>(t, 0, i))));
-      }
       dft_dif<4, INV>(x);
-      twist_mul<16>(x, tw0);
+#pragma unroll
+      for (int i = 1; i < 16; ++i) x[i] = gl::mul(x[i], tw0[i - 1]);
 #pragma unroll
       for (int i = 0; i < 16; ++i) lds[pidx(celem<LH, 0, 4>(t, 0, i))] = x[i];
       __syncthreads();
@@ -137,7 +181,7 @@ __global__ __launch_bounds__(NT) void ntt_contig_kernel(FastArgs a) {
       for (int i = 0; i < 16; ++i) x[i] = lds[pidx(celem<LH, 1, 4>(t, 0, i))];
       dft_dif<4, INV>(x);
       if constexpr (NL == 3) {
-        twist_mul<16>(x, tw1);
+        level_twist_lds<LH - 4>(x, tw1, rp1);
 #pragma unroll
         for (int i = 0; i < 16; ++i) lds[pidx(celem<LH, 1, 4>(t, 0, i))] = x[i];
         __syncthreads();
@@ -148,8 +192,7 @@ __global__ __launch_bounds__(NT) void ntt_contig_kernel(FastArgs a) {
 #pragma unroll
         for (int g = 0; g < G2; ++g) dft_dif<A2, INV>(x + g * (1 << A2));
       }
-      // store from the last level's registers
-      constexpr int LL = NL - 1, AL = NL == 3 ? A2 : 4, GL = 16 >> AL;
+      // store from the last level's registers (values are canonical)
 #pragma unroll
       for (int g = 0; g < GL; ++g)
 #pragma unroll
@@ -157,24 +200,20 @@ __global__ __launch_bounds__(NT) void ntt_contig_kernel(FastArgs a) {
           const uint32_t e = celem<LH, LL, AL>(t, g, i);
           uint64_t v = x[g * (1 << AL) + i];
           if constexpr (MODE == MODE_NINV) v = gl::mul(v, a.n_inv);
-          if constexpr (MODE == MODE_LDE_OUT) v = gl::mul(v, contig_scale<MODE, DIT>(a, pos_of(e)));
+          if constexpr (MODE == MODE_LDE_OUT) v = gl::mul(v, contig_scale<MODE, DIT>(a, ct.pos(e)));
           bool ok;
-          const uint64_t ad = addr_out(e, ok);
-          if (ok) a.out[ad] = gl::canon(v);  // n2b: the contiguous pass is the last one
+          const uint64_t ad = ct.addr(e, a.stride_out, ct.base_out, a.num_cols, ok);
+          if (ok) a.out[ad] = v;
         }
     } else {
-      constexpr int LL = NL - 1, AL = NL == 3 ? A2 : 4, GL = 16 >> AL;
+      // regroup the coalesced load into the last level's runs through LDS
+#pragma unroll
+      for (int i = 0; i < 16; ++i) lds[pidx(celem<LH, 0, 4>(t, 0, i))] = x[i];
+      __syncthreads();
 #pragma unroll
       for (int g = 0; g < GL; ++g)
 #pragma unroll
-        for (int i = 0; i < (1 << AL); ++i) {
-          const uint32_t e = celem<LH, LL, AL>(t, g, i);
-          bool ok;
-          const uint64_t ad = addr_in(e, ok);
-          uint64_t v = ok ? a.in[ad] : 0ull;
-          if constexpr (MODE == MODE_LDE_IN) v = gl::mul(v, contig_scale<MODE, DIT>(a, pos_of(e)));
-          x[g * (1 << AL) + i] = v;
-        }
+        for (int i = 0; i < (1 << AL); ++i) x[g * (1 << AL) + i] = lds[pidx(celem<LH, LL, AL>(t, g, i))];
       if constexpr (NL == 3) {
 #pragma unroll
         for (int g = 0; g < G2; ++g) dft_dit<A2, INV>(x + g * (1 << A2));
@@ -185,7 +224,7 @@ __global__ __launch_bounds__(NT) void ntt_contig_kernel(FastArgs a) {
         __syncthreads();
 #pragma unroll
         for (int i = 0; i < 16; ++i) x[i] = lds[pidx(celem<LH, 1, 4>(t, 0, i))];
-        twist_mul<16>(x, tw1);
+        level_twist_lds<LH - 4>(x, tw1, rp1);
       }
       dft_dit<4, INV>(x);
 #pragma unroll
@@ -193,17 +232,17 @@ __global__ __launch_bounds__(NT) void ntt_contig_kernel(FastArgs a) {
       __syncthreads();
 #pragma unroll
       for (int i = 0; i < 16; ++i) x[i] = lds[pidx(celem<LH, 0, 4>(t, 0, i))];
-      twist_mul<16>(x, tw0);
+#pragma unroll
+      for (int i = 1; i < 16; ++i) x[i] = gl::mul(x[i], tw0[i - 1]);
       dft_dit<4, INV>(x);
 #pragma unroll
       for (int i = 0; i < 16; ++i) {
         const uint32_t e = celem<LH, 0, 4>(t, 0, i);
         uint64_t v = x[i];
         if constexpr (MODE == MODE_NINV) v = gl::mul(v, a.n_inv);
-        if constexpr (MODE == MODE_LDE_OUT) v = gl::mul(v, contig_scale<MODE, DIT>(a, pos_of(e)));
-        if (a.last) v = gl::canon(v);
+        if constexpr (MODE == MODE_LDE_OUT) v = gl::mul(v, contig_scale<MODE, DIT>(a, ct.pos(e)));
         bool ok;
-        const uint64_t ad = addr_out(e, ok);
+        const uint64_t ad = ct.addr(e, a.stride_out, ct.base_out, a.num_cols, ok);
         if (ok) a.out[ad] = v;
       }
     }
@@ -222,7 +261,9 @@ __global__ __launch_bounds__(NT) void ntt_strided_kernel(FastArgs a) {
   constexpr int A1 = NL == 2 ? LH - 4 : 4;  // second level width
   constexpr int G1 = 16 >> A1;
   constexpr int LS0 = LH - 4;               // level-0 stride (log2)
-  __shared__ uint64_t lds[TILE];
+  constexpr uint32_t TW0_LEN = NL == 2 ? (15u << (LH - 4)) : 1u;
+  __shared__ uint64_t lds[NL == 2 ? TILE : 1];
+  __shared__ uint64_t tw0[TW0_LEN];
   const uint32_t t = threadIdx.x;
   const uint32_t log_r = a.log_N - LH;
   const uint32_t r0 = blockIdx.x << LT;
@@ -238,9 +279,11 @@ __global__ __launch_bounds__(NT) void ntt_strided_kernel(FastArgs a) {
     if constexpr (NL == 2) return (t + NT * (uint32_t)g) & (T - 1u);
     else return q0;
   };
+  // element offsets from the block base (r0 included) in each mapping
+  auto off0 = [&](int i) -> uint64_t { return ((uint64_t)(((uint32_t)i << LS0) + rp0) << log_r) + q0; };
+  auto offl = [&](int g, int i) -> uint64_t { return ((uint64_t)lastj(g, i) << log_r) + lastq(g); };
 
-  uint64_t tw0[15];
-  if constexpr (NL == 2) load_level_twists<INV>(a.tab, rp0, LH, tw0);
+  if constexpr (NL == 2) stage_level_twists<LH, INV>(tw0, a.tab);
 
   // pass twist omega_N^((r0 + q) * brev_h(j)) in the last-level mapping, with
   // n^-1 folded in when this pass carries the inverse scaling
@@ -275,23 +318,44 @@ __global__ __launch_bounds__(NT) void ntt_strided_kernel(FastArgs a) {
   const uint64_t total = (uint64_t)a.num_cols << blocks_per_col_log;
   const uint64_t p_begin = (uint64_t)blockIdx.y * a.blocks_per_wg;
   const uint64_t p_end = p_begin + a.blocks_per_wg < total ? p_begin + a.blocks_per_wg : total;
-  for (uint64_t p = p_begin; p < p_end; ++p) {
+  auto pair_off = [&](uint64_t p, uint64_t stride) -> uint64_t {
     const uint64_t col = p >> blocks_per_col_log;
     const uint64_t blk = p & ((1ull << blocks_per_col_log) - 1ull);
-    const uint64_t boff = (blk << a.log_N) + r0;
-    const uint64_t *in = a.in + col * a.stride_in + boff;
-    uint64_t *out = a.out + col * a.stride_out + boff;
-    uint64_t x[16];
-    if constexpr (!DIT) {
+    return col * stride + (blk << a.log_N) + r0;
+  };
+  // loads one pair's 16 values in the first level's mapping
+  auto load = [&](uint64_t p, uint64_t *v) {
+    const uint64_t *in = a.in + pair_off(p, a.stride_in);
 #pragma unroll
-      for (int i = 0; i < 16; ++i) x[i] = in[((uint64_t)(((uint32_t)i << LS0) + rp0) << log_r) + q0];
+    for (int g = 0; g < G1; ++g)
+#pragma unroll
+      for (int i = 0; i < (1 << A1); ++i) {
+        const int k = g * (1 << A1) + i;
+        if constexpr (!DIT) v[k] = in[off0(k)];
+        else v[k] = in[offl(g, i)];
+      }
+  };
+
+  uint64_t pf[16];
+  if (p_begin < p_end) load(p_begin, pf);
+  if constexpr (NL == 2) __syncthreads();  // tw0 staged
+  for (uint64_t p = p_begin; p < p_end; ++p) {
+    uint64_t x[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) x[i] = pf[i];
+    if (p + 1 < p_end) load(p + 1, pf);  // prefetch: in flight during this pair's math
+    uint64_t *out = a.out + pair_off(p, a.stride_out);
+    if constexpr (!DIT) {
       if constexpr (MODE == MODE_LDE_IN) {
 #pragma unroll
         for (int i = 0; i < 16; ++i) x[i] = gl::mul(x[i], cf[i]);
+      } else if (a.first) {
+#pragma unroll
+        for (int i = 0; i < 16; ++i) x[i] = gl::canon(x[i]);  // field ops take canonical values
       }
       dft_dif<4, INV>(x);
       if constexpr (NL == 2) {
-        twist_mul<16>(x, tw0);
+        level_twist_lds<LH>(x, tw0, rp0);
         // LDS row-major by j: element (q, j) at j * T + q
 #pragma unroll
         for (int i = 0; i < 16; ++i) lds[((((uint32_t)i << LS0) + rp0) << LT) + q0] = x[i];
@@ -309,16 +373,12 @@ __global__ __launch_bounds__(NT) void ntt_strided_kernel(FastArgs a) {
 #pragma unroll
         for (int i = 0; i < (1 << A1); ++i) {
           const int k = g * (1 << A1) + i;
-          out[((uint64_t)lastj(g, i) << log_r) + lastq(g)] = gl::mul(x[k], pt[k]);
+          out[offl(g, i)] = gl::mul(x[k], pt[k]);
         }
     } else {
+      // the pass twist makes every value canonical
 #pragma unroll
-      for (int g = 0; g < G1; ++g)
-#pragma unroll
-        for (int i = 0; i < (1 << A1); ++i) {
-          const int k = g * (1 << A1) + i;
-          x[k] = gl::mul(in[((uint64_t)lastj(g, i) << log_r) + lastq(g)], pt[k]);
-        }
+      for (int k = 0; k < 16; ++k) x[k] = gl::mul(x[k], pt[k]);
       if constexpr (NL == 2) {
 #pragma unroll
         for (int g = 0; g < G1; ++g) dft_dit<A1, INV>(x + g * (1 << A1));
@@ -330,15 +390,14 @@ __global__ __launch_bounds__(NT) void ntt_strided_kernel(FastArgs a) {
 #pragma unroll
         for (int i = 0; i < 16; ++i) x[i] = lds[((((uint32_t)i << LS0) + rp0) << LT) + q0];
         __syncthreads();
-        twist_mul<16>(x, tw0);
+        level_twist_lds<LH>(x, tw0, rp0);
       }
       dft_dit<4, INV>(x);
 #pragma unroll
       for (int i = 0; i < 16; ++i) {
         uint64_t v = x[i];
         if constexpr (MODE == MODE_LDE_OUT) v = gl::mul(v, cf[i]);
-        if (a.last) v = gl::canon(v);
-        out[((uint64_t)(((uint32_t)i << LS0) + rp0) << log_r) + q0] = v;
+        out[off0(i)] = v;
       }
     }
   }

@@ -18,6 +18,7 @@ constexpr uint32_t OMEGA_LOG_ORDER = 32;
 // so every omega_{2^k} power is an exponent E = e << (32 - k) in u32 arithmetic
 // (wrap-around is reduction mod 2^32, i.e. mod the subgroup order).
 struct Tables {
+  const uint64_t *lvl_tw;  // radix-16 level twists, see LVL_TW_* below
   const uint64_t *w_fine;
   const uint64_t *w_coarse;
   const uint64_t *g_fine;
@@ -28,6 +29,16 @@ struct Tables {
 constexpr uint32_t TABLE_LOG = 16;
 constexpr uint32_t TABLE_LEN = 1u << TABLE_LOG;
 constexpr uint32_t NUM_TABLES = 6;
+
+// Level twists of a radix-16 level inside a block of M = 2^m points
+// (5 <= m <= 12, stride S = M / 16): entry (i - 1) * S + rp holds
+// omega_M^(rp * brev4(i)) for i = 1..15, rp < S, so a wave reading one i for
+// consecutive rp reads consecutive words.  Tables for m = 5..12 are packed in
+// order, forward block first, then the inverse block (omega_M^-(...)).
+constexpr uint32_t LVL_TW_MIN_LOG = 5;
+constexpr uint32_t LVL_TW_MAX_LOG = 12;
+__host__ __device__ constexpr uint32_t lvl_tw_offset(uint32_t m) { return 15u * ((1u << (m - 4)) - 2u); }
+constexpr uint32_t LVL_TW_BLOCK = 15u * ((1u << (LVL_TW_MAX_LOG - 3)) - 2u);  // entries per direction
 
 // One HBM pass of a batched transform (see ntt_kernels.hip for the math).
 struct PassArgs {
@@ -60,6 +71,7 @@ struct FastArgs {
   uint32_t log_N;          // strided passes: block length this pass works inside
   uint32_t log_ext;
   uint32_t coset_exp;
+  uint32_t first;          // pass reads the caller's (possibly non-canonical) input
   uint32_t last;           // pass writes the final (canonical) output
   uint32_t tiles;          // contiguous passes: number of 4096-point tiles
   uint32_t blocks_per_wg;  // strided passes: (column, block) pairs per workgroup

@@ -149,6 +149,7 @@ __global__ __launch_bounds__(TILE_THREADS) void ntt_pass_kernel(PassArgs a) {
     uint64_t v = 0;
     if (el.valid) {
       v = a.in[(uint64_t)el.col * a.stride_in + el.pos];
+      if (a.first) v = gl::canon(v);  // the field ops below take canonical values
       if (!INV && a.first && a.log_ext) {
         // input position pos holds x_i with i = pos (n2b) or brev(pos) (b2n)
         const uint32_t idx = DIT ? brev_bits((uint32_t)el.pos, a.log_n) : (uint32_t)el.pos;
