@@ -79,14 +79,13 @@ __device__ __forceinline__ uint64_t contig_scale(const FastArgs &a, uint32_t pos
 }
 
 // Addressing of a contiguous-pass tile: 4096 consecutive points of one column,
-// or (single-pass transforms shorter than 4096) 2^(12-h) whole columns.
-template <int LH> struct ContigTile {
+// or (MC: single-pass transforms shorter than 4096) 2^(12-h) whole columns.
+template <int LH, bool MC> struct ContigTile {
   uint64_t base_in, base_out;
   uint32_t pos0, col0;
-  bool multicol;
+  static constexpr bool multicol = MC;
   __device__ __forceinline__ ContigTile(const FastArgs &a, uint32_t tile) {
-    multicol = a.log_n < TILE_LOG;
-    if (!multicol) {
+    if constexpr (!MC) {
       const uint32_t sh = a.log_n - TILE_LOG;
       const uint64_t col = tile >> sh;
       pos0 = (tile & ((1u << sh) - 1u)) << TILE_LOG;
@@ -100,7 +99,7 @@ template <int LH> struct ContigTile {
     }
   }
   __device__ __forceinline__ uint64_t addr(uint32_t e, uint64_t stride, uint64_t base, uint32_t ncols, bool &ok) const {
-    if (!multicol) {
+    if constexpr (!MC) {
       ok = true;
       return base + e;
     }
@@ -108,14 +107,17 @@ template <int LH> struct ContigTile {
     ok = c < ncols;
     return (uint64_t)c * stride + (e & ((1u << LH) - 1u));
   }
-  __device__ __forceinline__ uint32_t pos(uint32_t e) const { return multicol ? (e & ((1u << LH) - 1u)) : pos0 + e; }
+  __device__ __forceinline__ uint32_t pos(uint32_t e) const {
+    if constexpr (MC) return e & ((1u << LH) - 1u);
+    else return pos0 + e;
+  }
 };
 
 // Loads a tile's 16 values per thread in the level-0 mapping (lanes on
 // consecutive points).
-template <int LH>
+template <int LH, bool MC>
 __device__ __forceinline__ void contig_load(const FastArgs &a, uint32_t tile, uint32_t t, uint64_t *x) {
-  const ContigTile<LH> ct(a, tile);
+  const ContigTile<LH, MC> ct(a, tile);
 #pragma unroll
   for (int i = 0; i < 16; ++i) {
     bool ok;
@@ -124,7 +126,7 @@ __device__ __forceinline__ void contig_load(const FastArgs &a, uint32_t tile, ui
   }
 }
 
-template <int LH, bool INV, bool DIT, int MODE>
+template <int LH, bool INV, bool DIT, int MODE, bool MC>
 __global__ __launch_bounds__(NT) void ntt_contig_kernel(FastArgs a) {
   constexpr int NL = LH > 8 ? 3 : 2;
   constexpr int A2 = NL == 3 ? LH - 8 : 4;  // width of the third level
@@ -148,17 +150,17 @@ __global__ __launch_bounds__(NT) void ntt_contig_kernel(FastArgs a) {
 
   uint32_t tile = blockIdx.x;
   uint64_t pf[16];
-  if (tile < a.tiles) contig_load<LH>(a, tile, t, pf);
+  if (tile < a.tiles) contig_load<LH, MC>(a, tile, t, pf);
   __syncthreads();  // tw1 staged
 
   for (; tile < a.tiles; tile += gridDim.x) {
-    const ContigTile<LH> ct(a, tile);
+    const ContigTile<LH, MC> ct(a, tile);
     uint64_t x[16];
 #pragma unroll
     for (int i = 0; i < 16; ++i) x[i] = pf[i];
     {
       const uint32_t next = tile + gridDim.x;
-      if (next < a.tiles) contig_load<LH>(a, next, t, pf);  // prefetch: in flight during this tile's math
+      if (next < a.tiles) contig_load<LH, MC>(a, next, t, pf);  // prefetch: in flight during this tile's math
     }
     if constexpr (MODE == MODE_LDE_IN) {
 #pragma unroll
@@ -192,19 +194,24 @@ __global__ __launch_bounds__(NT) void ntt_contig_kernel(FastArgs a) {
 #pragma unroll
         for (int g = 0; g < G2; ++g) dft_dif<A2, INV>(x + g * (1 << A2));
       }
-      // store from the last level's registers (values are canonical)
+      // regroup the last level's runs into the coalesced level-0 mapping
+      // through LDS (each thread rewrites the slots it read, so no barrier
+      // is needed before the write), then store
 #pragma unroll
       for (int g = 0; g < GL; ++g)
 #pragma unroll
-        for (int i = 0; i < (1 << AL); ++i) {
-          const uint32_t e = celem<LH, LL, AL>(t, g, i);
-          uint64_t v = x[g * (1 << AL) + i];
-          if constexpr (MODE == MODE_NINV) v = gl::mul(v, a.n_inv);
-          if constexpr (MODE == MODE_LDE_OUT) v = gl::mul(v, contig_scale<MODE, DIT>(a, ct.pos(e)));
-          bool ok;
-          const uint64_t ad = ct.addr(e, a.stride_out, ct.base_out, a.num_cols, ok);
-          if (ok) a.out[ad] = v;
-        }
+        for (int i = 0; i < (1 << AL); ++i) lds[pidx(celem<LH, LL, AL>(t, g, i))] = x[g * (1 << AL) + i];
+      __syncthreads();
+#pragma unroll
+      for (int i = 0; i < 16; ++i) {
+        const uint32_t e = celem<LH, 0, 4>(t, 0, i);
+        uint64_t v = lds[pidx(e)];
+        if constexpr (MODE == MODE_NINV) v = gl::mul(v, a.n_inv);
+        if constexpr (MODE == MODE_LDE_OUT) v = gl::mul(v, contig_scale<MODE, DIT>(a, ct.pos(e)));
+        bool ok;
+        const uint64_t ad = ct.addr(e, a.stride_out, ct.base_out, a.num_cols, ok);
+        if (ok) a.out[ad] = v;  // canonical: the field ops keep values in [0, p)
+      }
     } else {
       // regroup the coalesced load into the last level's runs through LDS
 #pragma unroll
@@ -279,9 +286,15 @@ __global__ __launch_bounds__(NT) void ntt_strided_kernel(FastArgs a) {
     if constexpr (NL == 2) return (t + NT * (uint32_t)g) & (T - 1u);
     else return q0;
   };
-  // element offsets from the block base (r0 included) in each mapping
-  auto off0 = [&](int i) -> uint64_t { return ((uint64_t)(((uint32_t)i << LS0) + rp0) << log_r) + q0; };
-  auto offl = [&](int g, int i) -> uint64_t { return ((uint64_t)lastj(g, i) << log_r) + lastq(g); };
+  // Element offsets from the block base (r0 included).  Level-0 mapping:
+  // base0 + i * step0; last-level mapping: basel(g) + i * stepl.
+  const uint64_t step0 = 1ull << (LS0 + log_r);
+  const uint64_t base0 = ((uint64_t)rp0 << log_r) + q0;
+  const uint64_t stepl = NL == 2 ? (1ull << log_r) : step0;
+  auto basel = [&](int g) -> uint64_t {
+    if constexpr (NL == 2) return ((uint64_t)lastj(g, 0) << log_r) + lastq(g);
+    else return base0;
+  };
 
   if constexpr (NL == 2) stage_level_twists<LH, INV>(tw0, a.tab);
 
@@ -326,14 +339,18 @@ __global__ __launch_bounds__(NT) void ntt_strided_kernel(FastArgs a) {
   // loads one pair's 16 values in the first level's mapping
   auto load = [&](uint64_t p, uint64_t *v) {
     const uint64_t *in = a.in + pair_off(p, a.stride_in);
+    if constexpr (!DIT) {
+      const uint64_t *q = in + base0;
 #pragma unroll
-    for (int g = 0; g < G1; ++g)
+      for (int k = 0; k < 16; ++k, q += step0) v[k] = *q;
+    } else {
 #pragma unroll
-      for (int i = 0; i < (1 << A1); ++i) {
-        const int k = g * (1 << A1) + i;
-        if constexpr (!DIT) v[k] = in[off0(k)];
-        else v[k] = in[offl(g, i)];
+      for (int g = 0; g < G1; ++g) {
+        const uint64_t *q = in + basel(g);
+#pragma unroll
+        for (int i = 0; i < (1 << A1); ++i, q += stepl) v[g * (1 << A1) + i] = *q;
       }
+    }
   };
 
   uint64_t pf[16];
@@ -369,12 +386,14 @@ __global__ __launch_bounds__(NT) void ntt_strided_kernel(FastArgs a) {
         for (int g = 0; g < G1; ++g) dft_dif<A1, INV>(x + g * (1 << A1));
       }
 #pragma unroll
-      for (int g = 0; g < G1; ++g)
+      for (int g = 0; g < G1; ++g) {
+        uint64_t *q = out + basel(g);
 #pragma unroll
-        for (int i = 0; i < (1 << A1); ++i) {
+        for (int i = 0; i < (1 << A1); ++i, q += stepl) {
           const int k = g * (1 << A1) + i;
-          out[offl(g, i)] = gl::mul(x[k], pt[k]);
+          *q = gl::mul(x[k], pt[k]);
         }
+      }
     } else {
       // the pass twist makes every value canonical
 #pragma unroll
@@ -393,11 +412,12 @@ __global__ __launch_bounds__(NT) void ntt_strided_kernel(FastArgs a) {
         level_twist_lds<LH>(x, tw0, rp0);
       }
       dft_dit<4, INV>(x);
+      uint64_t *q = out + base0;
 #pragma unroll
-      for (int i = 0; i < 16; ++i) {
+      for (int i = 0; i < 16; ++i, q += step0) {
         uint64_t v = x[i];
         if constexpr (MODE == MODE_LDE_OUT) v = gl::mul(v, cf[i]);
-        out[off0(i)] = v;
+        *q = v;
       }
     }
   }
@@ -406,24 +426,24 @@ __global__ __launch_bounds__(NT) void ntt_strided_kernel(FastArgs a) {
 // -------------------------------------------------------------------------
 // launch dispatch (only the (direction, mode) pairs the planner can ask for)
 
-template <int LH, bool INV, bool DIT>
+template <int LH, bool INV, bool DIT, bool MC>
 hipError_t contig_mode(const FastArgs &a, int mode, dim3 grid, hipStream_t s) {
   switch (mode) {
-  case MODE_NONE: ntt_contig_kernel<LH, INV, DIT, MODE_NONE><<<grid, NT, 0, s>>>(a); break;
-  case MODE_NINV: ntt_contig_kernel<LH, INV, DIT, MODE_NINV><<<grid, NT, 0, s>>>(a); break;
-  case MODE_LDE_IN: ntt_contig_kernel<LH, INV, DIT, MODE_LDE_IN><<<grid, NT, 0, s>>>(a); break;
-  case MODE_LDE_OUT: ntt_contig_kernel<LH, INV, DIT, MODE_LDE_OUT><<<grid, NT, 0, s>>>(a); break;
+  case MODE_NONE: ntt_contig_kernel<LH, INV, DIT, MODE_NONE, MC><<<grid, NT, 0, s>>>(a); break;
+  case MODE_NINV: ntt_contig_kernel<LH, INV, DIT, MODE_NINV, MC><<<grid, NT, 0, s>>>(a); break;
+  case MODE_LDE_IN: ntt_contig_kernel<LH, INV, DIT, MODE_LDE_IN, MC><<<grid, NT, 0, s>>>(a); break;
+  case MODE_LDE_OUT: ntt_contig_kernel<LH, INV, DIT, MODE_LDE_OUT, MC><<<grid, NT, 0, s>>>(a); break;
   default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }
 
-template <int LH>
+template <int LH, bool MC>
 hipError_t contig_dir(const FastArgs &a, bool inv, bool dit, int mode, dim3 grid, hipStream_t s) {
   // forward transforms never scale outputs; inverse ones never scale inputs
   if (inv ? mode == MODE_LDE_IN : (mode == MODE_NINV || mode == MODE_LDE_OUT)) return hipErrorInvalidValue;
-  if (inv) return dit ? contig_mode<LH, true, true>(a, mode, grid, s) : contig_mode<LH, true, false>(a, mode, grid, s);
-  return dit ? contig_mode<LH, false, true>(a, mode, grid, s) : contig_mode<LH, false, false>(a, mode, grid, s);
+  if (inv) return dit ? contig_mode<LH, true, true, MC>(a, mode, grid, s) : contig_mode<LH, true, false, MC>(a, mode, grid, s);
+  return dit ? contig_mode<LH, false, true, MC>(a, mode, grid, s) : contig_mode<LH, false, false, MC>(a, mode, grid, s);
 }
 
 template <int LH, bool INV, bool DIT>
@@ -456,12 +476,20 @@ hipError_t launch_contig(const FastArgs &a, int log_h, bool inverse, bool dit, i
   if (a.tiles == 0) return hipSuccess;
   const uint32_t max_wgs = 2048;
   const dim3 grid(a.tiles < max_wgs ? a.tiles : max_wgs);
+  if (a.log_n < TILE_LOG) {  // single pass over whole columns, several per tile
+    switch (log_h) {
+    case 8: return contig_dir<8, true>(a, inverse, dit, mode, grid, stream);
+    case 9: return contig_dir<9, true>(a, inverse, dit, mode, grid, stream);
+    case 10: return contig_dir<10, true>(a, inverse, dit, mode, grid, stream);
+    case 11: return contig_dir<11, true>(a, inverse, dit, mode, grid, stream);
+    default: return hipErrorInvalidValue;
+    }
+  }
   switch (log_h) {
-  case 8: return contig_dir<8>(a, inverse, dit, mode, grid, stream);
-  case 9: return contig_dir<9>(a, inverse, dit, mode, grid, stream);
-  case 10: return contig_dir<10>(a, inverse, dit, mode, grid, stream);
-  case 11: return contig_dir<11>(a, inverse, dit, mode, grid, stream);
-  case 12: return contig_dir<12>(a, inverse, dit, mode, grid, stream);
+  case 9: return contig_dir<9, false>(a, inverse, dit, mode, grid, stream);
+  case 10: return contig_dir<10, false>(a, inverse, dit, mode, grid, stream);
+  case 11: return contig_dir<11, false>(a, inverse, dit, mode, grid, stream);
+  case 12: return contig_dir<12, false>(a, inverse, dit, mode, grid, stream);
   default: return hipErrorInvalidValue;
   }
 }
