@@ -23,6 +23,11 @@ SIGNATURES = {
                                           c_bool, c_bool, c_u32, c_u32, c_vp]),
     "bc_batch_ntt_in_place": (c_int, [c_vp, c_size, c_u32, c_u32, c_u32, c_u32, c_bool, c_bool, c_u32, c_u32,
                                       c_vp]),
+    "bc_batch_lde": (c_int, [c_vp, c_vp, c_vp, c_u32, c_u32, c_u32, c_u32, c_u64, c_u32, c_bool, c_vp]),
+    "bc_lde_graph_create": (c_int, [c_vp, c_vp, c_vp, c_u32, c_u32, c_u32, c_u32, c_u64, c_u32, c_bool,
+                                    ctypes.POINTER(c_vp)]),
+    "bc_graph_launch": (c_int, [c_vp, c_vp]),
+    "bc_graph_destroy": (c_int, [c_vp]),
 }
 
 BC_SUCCESS = 0

@@ -288,3 +288,92 @@ extern "C" int bc_batch_ntt_in_place(uint64_t *inputs_matrix, size_t inputs_len,
   return bc_batch_ntt_internal(p, p, log_n, num_ntts, stride, stride, bitrev_inputs, inverse, log_ext, coset_index,
                                stream);
 }
+
+// ---------------------------------------------------------------------------
+// Coset LDE pipeline and graph capture
+
+namespace bc {
+namespace {
+uint32_t brev_small(uint32_t x, uint32_t bits) {
+  uint32_t r = 0;
+  for (uint32_t i = 0; i < bits; ++i) r |= ((x >> i) & 1u) << (bits - 1 - i);
+  return r;
+}
+}  // namespace
+}  // namespace bc
+
+extern "C" int bc_batch_lde(const uint64_t *inputs, uint64_t *coeffs, uint64_t *outputs, uint32_t log_n,
+                            uint32_t num_cols, uint32_t stride_in, uint32_t stride_coeffs, uint64_t stride_out,
+                            uint32_t log_blowup, bool bitrev_inputs, void *stream) {
+  using namespace bc;
+  if (!inputs || !coeffs || !outputs || log_n < 1 || log_n >= 32) return BC_ERROR_INVALID_VALUE;
+  if ((uint64_t)log_n + log_blowup > OMEGA_LOG_ORDER || log_blowup > 16) return BC_ERROR_INVALID_VALUE;
+  const uint64_t n = 1ull << log_n;
+  if (stride_out < (n << log_blowup) || stride_out > 0xFFFFFFFFull) return BC_ERROR_INVALID_VALUE;
+  // 1. evaluations -> coefficients (natural order when the input is
+  //    bit-reversed, bit-reversed otherwise)
+  int rc = bc_batch_ntt_internal(inputs, coeffs, log_n, num_cols, stride_in, stride_coeffs, bitrev_inputs, true, 0, 0,
+                                 stream);
+  if (rc != BC_SUCCESS) return rc;
+  // 2. one forward coset transform per coset; natural coefficients give
+  //    bit-reversed evaluations (n2b) and vice versa (b2n)
+  for (uint32_t c = 0; c < (1u << log_blowup); ++c) {
+    const uint32_t blk = bitrev_inputs ? brev_small(c, log_blowup) : c;
+    rc = bc_batch_ntt_internal(coeffs, outputs + (uint64_t)blk * n, log_n, num_cols, stride_coeffs,
+                               (uint32_t)stride_out, !bitrev_inputs, false, log_blowup, c, stream);
+    if (rc != BC_SUCCESS) return rc;
+  }
+  return BC_SUCCESS;
+}
+
+struct bc_graph {
+  hipGraph_t graph;
+  hipGraphExec_t exec;
+};
+
+extern "C" int bc_lde_graph_create(const uint64_t *inputs, uint64_t *coeffs, uint64_t *outputs, uint32_t log_n,
+                                   uint32_t num_cols, uint32_t stride_in, uint32_t stride_coeffs,
+                                   uint64_t stride_out, uint32_t log_blowup, bool bitrev_inputs, bc_graph **out) {
+  if (!out) return BC_ERROR_INVALID_VALUE;
+  *out = nullptr;
+  hipStream_t cs;
+  hipError_t err = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
+  if (err != hipSuccess) return err;
+  err = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+  if (err != hipSuccess) {
+    (void)hipStreamDestroy(cs);
+    return err;
+  }
+  const int rc = bc_batch_lde(inputs, coeffs, outputs, log_n, num_cols, stride_in, stride_coeffs, stride_out,
+                              log_blowup, bitrev_inputs, cs);
+  hipGraph_t g = nullptr;
+  err = hipStreamEndCapture(cs, &g);
+  (void)hipStreamDestroy(cs);
+  if (rc != BC_SUCCESS) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (err != hipSuccess) return err;
+  hipGraphExec_t ex = nullptr;
+  err = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+  if (err != hipSuccess) {
+    (void)hipGraphDestroy(g);
+    return err;
+  }
+  bc_graph *h = new bc_graph{g, ex};
+  *out = h;
+  return BC_SUCCESS;
+}
+
+extern "C" int bc_graph_launch(bc_graph *graph, void *stream) {
+  if (!graph) return BC_ERROR_INVALID_VALUE;
+  return hipGraphLaunch(graph->exec, (hipStream_t)stream);
+}
+
+extern "C" int bc_graph_destroy(bc_graph *graph) {
+  if (!graph) return BC_ERROR_INVALID_VALUE;
+  hipError_t e1 = hipGraphExecDestroy(graph->exec);
+  hipError_t e2 = hipGraphDestroy(graph->graph);
+  delete graph;
+  return e1 != hipSuccess ? e1 : e2;
+}

@@ -72,6 +72,39 @@ int bc_batch_ntt_in_place(uint64_t *inputs_matrix, size_t inputs_len, uint32_t l
                           uint32_t inputs_offset, uint32_t stride_between_input_arrays, bool bitrev_inputs,
                           bool inverse, uint32_t log_extension_degree, uint32_t coset_index, void *stream);
 
+/* ---- Extensions beyond the reference API (north-star pipeline pieces) ---- */
+
+/* Coset low-degree extension of a batch of columns.  No single reference
+ * function does this; it chains the reference's own calls the way its users
+ * do (src/ntt.rs:176-177 describes the intent): an inverse transform of the
+ * evaluations on H to coefficients, then one forward coset transform
+ * (log_extension_degree = log_blowup, coset_index = c) per coset.
+ *   inputs:  evaluations, column i at inputs + i * stride_in (n values,
+ *            bit-reversed order if bitrev_inputs, else natural)
+ *   coeffs:  receives the coefficients (n per column, stride stride_coeffs);
+ *            may equal inputs (then the inputs are overwritten)
+ *   outputs: column i at outputs + i * stride_out holds 2^log_blowup blocks
+ *            of n values.  With bitrev_inputs, block brev(c) holds coset c in
+ *            bit-reversed order, so the column is the bit-reversed LDE over
+ *            g * H_{n 2^log_blowup}; otherwise block c holds coset c in
+ *            natural order.
+ * stride_out >= n << log_blowup; returns as bc_batch_ntt_internal. */
+int bc_batch_lde(const uint64_t *inputs, uint64_t *coeffs, uint64_t *outputs, uint32_t log_n, uint32_t num_cols,
+                 uint32_t stride_in, uint32_t stride_coeffs, uint64_t stride_out, uint32_t log_blowup,
+                 bool bitrev_inputs, void *stream);
+
+/* A captured, replayable launch sequence (hipGraph) for a fixed shape. */
+typedef struct bc_graph bc_graph;
+
+/* Captures bc_batch_lde with these exact arguments into a graph (on an
+ * internal stream; nothing runs during capture).  Launch it with
+ * bc_graph_launch on any stream; buffers must stay valid. */
+int bc_lde_graph_create(const uint64_t *inputs, uint64_t *coeffs, uint64_t *outputs, uint32_t log_n,
+                        uint32_t num_cols, uint32_t stride_in, uint32_t stride_coeffs, uint64_t stride_out,
+                        uint32_t log_blowup, bool bitrev_inputs, bc_graph **out);
+int bc_graph_launch(bc_graph *graph, void *stream);
+int bc_graph_destroy(bc_graph *graph);
+
 #ifdef __cplusplus
 }
 #endif
