@@ -39,6 +39,40 @@ def uniform_field(count, gen):
     return torch.where(big, x + (2**32 - 1), x)
 
 
+def bench_lde(log_n, cols, log_blowup, reps, warmup):
+    """Coset LDE: bit-reversed evaluations -> coefficients -> 2^k cosets,
+    replayed from a captured hipGraph.  Algorithmic bytes: 8 * (1 + B) * n * cols."""
+    import torch
+    from boojum_cuda_amd import lde
+    n = 1 << log_n
+    g = torch.Generator(device="cuda")
+    g.manual_seed(3)
+    evals = uniform_field(cols * n, g)
+    coeffs = torch.empty_like(evals)
+    out = torch.empty(cols * (n << log_blowup), dtype=torch.int64, device="cuda")
+    graph = lde.LdeGraph(evals, coeffs, out, log_n, cols, n, n, n << log_blowup, log_blowup, True)
+    stream = torch.cuda.current_stream()
+    for _ in range(warmup):
+        graph.launch()
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2 * reps)]
+    for r in range(reps):
+        ev[2 * r].record(stream)
+        graph.launch()
+        ev[2 * r + 1].record(stream)
+    torch.cuda.synchronize()
+    ms = sum(ev[2 * r].elapsed_time(ev[2 * r + 1]) for r in range(reps)) / reps
+    graph.destroy()
+    blowup = 1 << log_blowup
+    algo = 8.0 * (1 + blowup) * n * cols
+    del evals, coeffs, out
+    return {"workload": f"coset LDE {cols} x 2^{log_n}, blowup {blowup}, bit-reversed in/out, hipGraph",
+            "ms_per_call": ms, "lde_elems_per_s": blowup * n * cols / (ms * 1e-3),
+            "roofline": {"bound": "hbm", "achieved": algo / (ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": algo / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                         "algorithmic_bytes_per_call": algo}}
+
+
 def cpu_baseline(log_n, cols, threads):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import numpy as np
@@ -62,6 +96,7 @@ def main():
     ap.add_argument("--cols", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-cols", type=int, default=64)
+    ap.add_argument("--no-lde", action="store_true", help="skip the coset-LDE measurement")
     ap.add_argument("--mode", choices=["ntt", "copy"], default="ntt",
                     help="copy: time a plain device copy of the same matrix (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -125,6 +160,10 @@ def main():
     value = elems_per_step / (ms_per_step * 1e-3)
     algo_bytes = 16.0 * cols * n
     achieved = algo_bytes / (avg_call_ms * 1e-3) / 1e9
+    del x, y
+    extras = {}
+    if not args.no_lde:
+        extras["lde"] = bench_lde(20, 128, 2, max(3, args.steps // 2), 2)
     ctx.destroy()
 
     rec = {
@@ -147,6 +186,7 @@ def main():
                      "kernel": "one batch_ntt call (all its pass kernels)", "avg_call_ms": avg_call_ms,
                      "algorithmic_bytes_per_call": algo_bytes},
     }
+    rec.update(extras)
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         rec["cpu_baseline"] = cpu_baseline(log_n, args.cpu_cols, threads)

@@ -93,6 +93,78 @@ int run_general(bc_context *ctx, const uint64_t *inputs, uint64_t *outputs, uint
 }
 
 }  // namespace
+
+Tables current_tables(bool &ok) {
+  bc_context *ctx = current_context();
+  ok = ctx != nullptr;
+  return ok ? ctx->tab : Tables{};
+}
+
+}  // namespace bc
+
+namespace bc {
+
+int describe_fast_passes(const Tables &tab, const uint64_t *inputs, uint64_t *outputs, uint32_t log_n,
+                         uint32_t num_cols, uint64_t stride_in, uint64_t stride_out, bool bitrev_inputs, bool inverse,
+                         uint32_t log_ext, uint32_t coset_index, PassDesc *out) {
+  if (log_n < 8) return -1;
+  FastPlan plan;
+  make_fast_plan(log_n, plan);
+  const uint64_t n = 1ull << log_n;
+  FastArgs a{};
+  a.tab = tab;
+  a.num_cols = num_cols;
+  a.log_n = log_n;
+  a.log_ext = log_ext;
+  a.coset_exp = log_ext ? (uint32_t)((uint64_t)coset_index << (32 - log_n - log_ext)) : 0u;
+  a.n_inv = gl::inv(n % gl::P);
+  const int np = plan.num_strided + 1;
+  const bool single = np == 1;
+  for (int k = 0; k < np; ++k) {
+    // n2b runs strided passes top-down then the contiguous pass; b2n the reverse
+    const int p = bitrev_inputs ? (np - 1 - k) : k;  // index in n2b order
+    const bool first = k == 0, last = k == np - 1;
+    PassDesc &pd = out[k];
+    pd.a = a;
+    pd.a.in = first ? inputs : outputs;
+    pd.a.stride_in = first ? stride_in : stride_out;
+    pd.a.out = outputs;
+    pd.a.stride_out = stride_out;
+    pd.a.first = first;
+    pd.a.last = last;
+    pd.positions = pd.pairs = 0;
+    if (p == np - 1) {
+      int mode = MODE_NONE;
+      if (!inverse && log_ext && first) mode = MODE_LDE_IN;
+      else if (inverse && last && log_ext) mode = MODE_LDE_OUT;
+      else if (inverse && last && single) mode = MODE_NINV;
+      pd.contig = true;
+      pd.log_h = plan.contig;
+      pd.mode = mode;
+      pd.a.log_N = plan.contig;
+      const uint64_t tiles = log_n >= TILE_LOG ? ((uint64_t)num_cols << (log_n - TILE_LOG))
+                                               : (num_cols + (1ull << (TILE_LOG - log_n)) - 1) >> (TILE_LOG - log_n);
+      if (tiles > 0xFFFFFFFFull) return -1;
+      pd.a.tiles = (uint32_t)tiles;
+    } else {
+      const bool top = p == 0;
+      int mode = MODE_NONE;
+      if (top) {
+        if (!inverse && log_ext && !bitrev_inputs) mode = MODE_LDE_IN;
+        else if (inverse && log_ext && bitrev_inputs) mode = MODE_LDE_OUT;
+        else if (inverse && !log_ext) mode = MODE_NINV;
+      }
+      pd.contig = false;
+      pd.log_h = plan.strided[p];
+      pd.mode = mode;
+      pd.a.log_N = plan.log_N[p];
+      pd.positions = 1ull << (pd.a.log_N - TILE_LOG);
+      pd.pairs = (uint64_t)num_cols << (log_n - pd.a.log_N);
+    }
+  }
+  return np;
+}
+
 }  // namespace bc
 
 extern "C" uint32_t bc_omega_log_order(void) { return bc::OMEGA_LOG_ORDER; }
@@ -192,63 +264,29 @@ extern "C" int bc_batch_ntt_internal(const uint64_t *inputs, uint64_t *outputs, 
   if (!ctx) return BC_ERROR_NOT_INITIALIZED;
 
   hipStream_t s = (hipStream_t)stream;
-  const uint32_t coset_exp = log_ext ? (uint32_t)((uint64_t)coset_index << (32 - log_n - log_ext)) : 0u;
-  const uint64_t n_inv = gl::inv(n % gl::P);
+  if (log_n < 8) {
+    const uint32_t coset_exp = log_ext ? (uint32_t)((uint64_t)coset_index << (32 - log_n - log_ext)) : 0u;
+    return run_general(ctx, inputs, outputs, log_n, num_ntts, stride_in, stride_out, bitrev_inputs, inverse, log_ext,
+                       coset_exp, gl::inv(n % gl::P), s);
+  }
 
-  if (log_n < 8) return run_general(ctx, inputs, outputs, log_n, num_ntts, stride_in, stride_out, bitrev_inputs,
-                                    inverse, log_ext, coset_exp, n_inv, s);
-
-  FastPlan plan;
-  make_fast_plan(log_n, plan);
-  FastArgs a{};
-  a.tab = ctx->tab;
-  a.num_cols = num_ntts;
-  a.log_n = log_n;
-  a.log_ext = log_ext;
-  a.coset_exp = coset_exp;
-  a.n_inv = n_inv;
-  const int np = plan.num_strided + 1;
-  const bool single = np == 1;
+  PassDesc passes[8];
+  const int np = describe_fast_passes(ctx->tab, inputs, outputs, log_n, num_ntts, stride_in, stride_out,
+                                      bitrev_inputs, inverse, log_ext, coset_index, passes);
+  if (np <= 0) return BC_ERROR_INVALID_VALUE;
   for (int k = 0; k < np; ++k) {
-    // n2b runs strided passes top-down then the contiguous pass; b2n the reverse
-    const int p = bitrev_inputs ? (np - 1 - k) : k;  // index in n2b order
-    const bool first = k == 0, last = k == np - 1;
-    a.in = first ? inputs : outputs;
-    a.stride_in = first ? stride_in : stride_out;
-    a.out = outputs;
-    a.stride_out = stride_out;
-    a.first = first;
-    a.last = last;
+    PassDesc &pd = passes[k];
     hipError_t err;
-    if (p == np - 1) {
-      int mode = MODE_NONE;
-      if (!inverse && log_ext && first) mode = MODE_LDE_IN;
-      else if (inverse && last && log_ext) mode = MODE_LDE_OUT;
-      else if (inverse && last && single) mode = MODE_NINV;
-      a.log_N = plan.contig;
-      const uint64_t tiles = ((uint64_t)num_ntts << log_n) >> TILE_LOG;
-      const uint64_t tiles_ceil = log_n >= TILE_LOG ? tiles : (num_ntts + (1u << (TILE_LOG - log_n)) - 1) >> (TILE_LOG - log_n);
-      if (tiles_ceil > 0xFFFFFFFFull) return BC_ERROR_INVALID_VALUE;
-      a.tiles = (uint32_t)tiles_ceil;
-      err = launch_contig(a, plan.contig, inverse, bitrev_inputs, mode, s);
+    if (pd.contig) {
+      err = launch_contig(pd.a, (int)pd.log_h, inverse, bitrev_inputs, pd.mode, s);
     } else {
-      const bool top = p == 0;
-      int mode = MODE_NONE;
-      if (top) {
-        if (!inverse && log_ext && !bitrev_inputs) mode = MODE_LDE_IN;
-        else if (inverse && log_ext && bitrev_inputs) mode = MODE_LDE_OUT;
-        else if (inverse && !log_ext) mode = MODE_NINV;
-      }
-      const uint32_t lh = plan.strided[p];
-      a.log_N = plan.log_N[p];
-      const uint64_t positions = 1ull << (a.log_N - TILE_LOG);
-      const uint64_t pairs = (uint64_t)num_ntts << (log_n - a.log_N);
-      uint64_t bpw = pairs * positions / 2048;
+      uint64_t bpw = pd.pairs * pd.positions / 2048;
       bpw = bpw < 1 ? 1 : (bpw > 16 ? 16 : bpw);
-      const uint64_t groups = (pairs + bpw - 1) / bpw;
-      if (positions > 0x7FFFFFFFull || groups > 65535) return BC_ERROR_INVALID_VALUE;
-      a.blocks_per_wg = (uint32_t)bpw;
-      err = launch_strided(a, (int)lh, inverse, bitrev_inputs, mode, (uint32_t)positions, (uint32_t)groups, s);
+      const uint64_t groups = (pd.pairs + bpw - 1) / bpw;
+      if (pd.positions > 0x7FFFFFFFull || groups > 65535) return BC_ERROR_INVALID_VALUE;
+      pd.a.blocks_per_wg = (uint32_t)bpw;
+      err = launch_strided(pd.a, (int)pd.log_h, inverse, bitrev_inputs, pd.mode, (uint32_t)pd.positions,
+                           (uint32_t)groups, s);
     }
     if (err != hipSuccess) return err;
   }

@@ -1,426 +1,24 @@
-// Register-resident pass kernels for the batched NTT on gfx950.
-//
-// The math of a pass is the one described in ntt_kernels.hip: inside blocks of
-// N points, for every r < R = N / H, an H-point sub-DFT over j (positions
-// j * R + r), with the twist omega_N^(r * brev(j)).  Two tile shapes cover
-// every pass of a transform with log_n >= 8:
-//
-//   contig<h>  (8 <= h <= 12): R = 1, the last pass in n2b order.  A tile is
-//              4096 consecutive points = 2^(12-h) sub-DFTs.  Levels of radix
-//              16, 16 and 2^(h-8) run in registers, two LDS exchanges.
-//   strided<h> (4 <= h <= 8):  R >= 16, every other pass.  A tile is
-//              2^(12-h) consecutive r times the h-point sub-DFT, so each row of
-//              the tile is >= 128 contiguous bytes.  Levels of radix 16 and
-//              2^(h-4), one LDS exchange (none for h = 4).
-//
-// Every workgroup has 256 threads holding 16 values each.  Global loads go
-// straight into the first level's registers and the last level stores from
-// registers.  Per-thread level twists depend only on the thread and are
-// computed once per workgroup; the strided pass twist depends on the tile's r
-// range, so a workgroup walks several (column, block) pairs that share it and
-// computes it once.  n2b runs levels top-down (DIF), b2n bottom-up (DIT) as
-// the transpose.
-#include "ntt_device.hpp"
+// Standalone launches of the NTT pass kernels (one kernel per pass); the
+// pass bodies and their math are in ntt_passes.hpp.
+#include "ntt_passes.hpp"
 
 namespace bc {
 namespace {
-
-constexpr uint32_t NT = 256;  // threads per workgroup
-
-// Padded LDS index: one pad slot per 16 keeps stride-16 accesses on distinct banks.
-__device__ __forceinline__ uint32_t pidx(uint32_t e) { return e + (e >> 4); }
-constexpr uint32_t LDS_PADDED = TILE + TILE / 16;
-
-// Level-twist table of a radix-16 level in blocks of M = 2^LM points: entry
-// (i - 1) * S + rp = omega_M^(+-rp * brev4(i)), S = M / 16.
-template <int LM, bool INV> __device__ __forceinline__ const uint64_t *lvl_table(const Tables &tab) {
-  return tab.lvl_tw + (INV ? LVL_TW_BLOCK : 0u) + lvl_tw_offset(LM);
-}
-
-// Copies a level-twist table into LDS (once per workgroup).
-template <int LM, bool INV> __device__ __forceinline__ void stage_level_twists(uint64_t *dst, const Tables &tab) {
-  constexpr uint32_t LEN = 15u << (LM - 4);
-  const uint64_t *src = lvl_table<LM, INV>(tab);
-  for (uint32_t k = threadIdx.x; k < LEN; k += NT) dst[k] = src[k];
-}
-
-// x[i] *= tw[(i - 1) * S + rp] for i = 1..15 (tw in LDS).
-template <int LM>
-__device__ __forceinline__ void level_twist_lds(uint64_t *x, const uint64_t *tw, uint32_t rp) {
-  constexpr uint32_t S = 1u << (LM - 4);
-#pragma unroll
-  for (int i = 1; i < 16; ++i) x[i] = gl::mul(x[i], tw[(i - 1) * S + rp]);
-}
-
-// -------------------------------------------------------------------------
-// contig<h>
-
-// Tile element e of thread t's group g, point i, at level L (width A) of an
-// h-point sub-DFT whose levels are 4, 4, h - 8.
-template <int LH, int L, int A> __device__ __forceinline__ uint32_t celem(uint32_t t, int g, int i) {
-  constexpr int C = 4 * L, LM = LH - C, LS = LM - A;
-  const uint32_t gi = t + NT * (uint32_t)g;
-  const uint32_t rp = gi & ((1u << LS) - 1u);
-  const uint32_t b = (gi >> LS) & ((1u << C) - 1u);
-  const uint32_t qb = gi >> (LH - A);
-  return (qb << LH) + (b << LM) + ((uint32_t)i << LS) + rp;
-}
-
-// Scaling factor of a contiguous-pass element at column position pos.
-template <int MODE, bool DIT>
-__device__ __forceinline__ uint64_t contig_scale(const FastArgs &a, uint32_t pos) {
-  if constexpr (MODE == MODE_LDE_IN) {
-    const uint32_t idx = DIT ? brev_bits(pos, a.log_n) : pos;  // input position holds x_idx
-    return coset_pow(a.tab, a.coset_exp, idx, false);
-  } else {
-    const uint32_t idx = DIT ? pos : brev_bits(pos, a.log_n);  // output position holds y_idx
-    return gl::mul(a.n_inv, coset_pow(a.tab, a.coset_exp, idx, true));
-  }
-}
-
-// Addressing of a contiguous-pass tile: 4096 consecutive points of one column,
-// or (MC: single-pass transforms shorter than 4096) 2^(12-h) whole columns.
-template <int LH, bool MC> struct ContigTile {
-  uint64_t base_in, base_out;
-  uint32_t pos0, col0;
-  static constexpr bool multicol = MC;
-  __device__ __forceinline__ ContigTile(const FastArgs &a, uint32_t tile) {
-    if constexpr (!MC) {
-      const uint32_t sh = a.log_n - TILE_LOG;
-      const uint64_t col = tile >> sh;
-      pos0 = (tile & ((1u << sh) - 1u)) << TILE_LOG;
-      base_in = col * a.stride_in + pos0;
-      base_out = col * a.stride_out + pos0;
-      col0 = 0;
-    } else {
-      col0 = tile << (12 - LH);
-      pos0 = 0;
-      base_in = base_out = 0;
-    }
-  }
-  __device__ __forceinline__ uint64_t addr(uint32_t e, uint64_t stride, uint64_t base, uint32_t ncols, bool &ok) const {
-    if constexpr (!MC) {
-      ok = true;
-      return base + e;
-    }
-    const uint32_t c = col0 + (e >> LH);
-    ok = c < ncols;
-    return (uint64_t)c * stride + (e & ((1u << LH) - 1u));
-  }
-  __device__ __forceinline__ uint32_t pos(uint32_t e) const {
-    if constexpr (MC) return e & ((1u << LH) - 1u);
-    else return pos0 + e;
-  }
-};
-
-// Loads a tile's 16 values per thread in the level-0 mapping (lanes on
-// consecutive points).
-template <int LH, bool MC>
-__device__ __forceinline__ void contig_load(const FastArgs &a, uint32_t tile, uint32_t t, uint64_t *x) {
-  const ContigTile<LH, MC> ct(a, tile);
-#pragma unroll
-  for (int i = 0; i < 16; ++i) {
-    bool ok;
-    const uint64_t ad = ct.addr(celem<LH, 0, 4>(t, 0, i), a.stride_in, ct.base_in, a.num_cols, ok);
-    x[i] = ok ? a.in[ad] : 0ull;
-  }
-}
+using namespace pass;
 
 template <int LH, bool INV, bool DIT, int MODE, bool MC>
 __global__ __launch_bounds__(NT) void ntt_contig_kernel(FastArgs a) {
-  constexpr int NL = LH > 8 ? 3 : 2;
-  constexpr int A2 = NL == 3 ? LH - 8 : 4;  // width of the third level
-  constexpr int G2 = 16 >> A2;              // third-level groups per thread
-  constexpr int LL = NL - 1, AL = NL == 3 ? A2 : 4, GL = 16 >> AL;  // last level
-  constexpr uint32_t TW1_LEN = NL == 3 ? (15u << (LH - 8)) : 1u;
-  __shared__ uint64_t lds[LDS_PADDED];
-  __shared__ uint64_t tw1[TW1_LEN];
-  const uint32_t t = threadIdx.x;
-
-  // level-0 twists: 15 per thread, in registers for the whole kernel
-  uint64_t tw0[15];
-  {
-    const uint64_t *src = lvl_table<LH, INV>(a.tab) + (t & ((1u << (LH - 4)) - 1u));
-#pragma unroll
-    for (int i = 1; i < 16; ++i) tw0[i - 1] = src[(i - 1) << (LH - 4)];
-  }
-  // level-1 twists (at most 240 entries) in LDS
-  if constexpr (NL == 3) stage_level_twists<LH - 4, INV>(tw1, a.tab);
-  const uint32_t rp1 = t & ((1u << (LH - 8)) - 1u);
-
-  uint32_t tile = blockIdx.x;
-  uint64_t pf[16];
-  if (tile < a.tiles) contig_load<LH, MC>(a, tile, t, pf);
-  __syncthreads();  // tw1 staged
-
-  for (; tile < a.tiles; tile += gridDim.x) {
-    const ContigTile<LH, MC> ct(a, tile);
-    uint64_t x[16];
-#pragma unroll
-    for (int i = 0; i < 16; ++i) x[i] = pf[i];
-    {
-      const uint32_t next = tile + gridDim.x;
-      if (next < a.tiles) contig_load<LH, MC>(a, next, t, pf);  // prefetch: in flight during this tile's math
-    }
-    if constexpr (MODE == MODE_LDE_IN) {
-#pragma unroll
-      for (int i = 0; i < 16; ++i) x[i] = gl::mul(x[i], contig_scale<MODE, DIT>(a, ct.pos(celem<LH, 0, 4>(t, 0, i))));
-    } else {
-      if (a.first) {
-#pragma unroll
-        for (int i = 0; i < 16; ++i) x[i] = gl::canon(x[i]);  // field ops take canonical values
-      }
-    }
-
-    if constexpr (!DIT) {
-      dft_dif<4, INV>(x);
-#pragma unroll
-      for (int i = 1; i < 16; ++i) x[i] = gl::mul(x[i], tw0[i - 1]);
-#pragma unroll
-      for (int i = 0; i < 16; ++i) lds[pidx(celem<LH, 0, 4>(t, 0, i))] = x[i];
-      __syncthreads();
-#pragma unroll
-      for (int i = 0; i < 16; ++i) x[i] = lds[pidx(celem<LH, 1, 4>(t, 0, i))];
-      dft_dif<4, INV>(x);
-      if constexpr (NL == 3) {
-        level_twist_lds<LH - 4>(x, tw1, rp1);
-#pragma unroll
-        for (int i = 0; i < 16; ++i) lds[pidx(celem<LH, 1, 4>(t, 0, i))] = x[i];
-        __syncthreads();
-#pragma unroll
-        for (int g = 0; g < G2; ++g)
-#pragma unroll
-          for (int i = 0; i < (1 << A2); ++i) x[g * (1 << A2) + i] = lds[pidx(celem<LH, 2, A2>(t, g, i))];
-#pragma unroll
-        for (int g = 0; g < G2; ++g) dft_dif<A2, INV>(x + g * (1 << A2));
-      }
-      // regroup the last level's runs into the coalesced level-0 mapping
-      // through LDS (each thread rewrites the slots it read, so no barrier
-      // is needed before the write), then store
-#pragma unroll
-      for (int g = 0; g < GL; ++g)
-#pragma unroll
-        for (int i = 0; i < (1 << AL); ++i) lds[pidx(celem<LH, LL, AL>(t, g, i))] = x[g * (1 << AL) + i];
-      __syncthreads();
-#pragma unroll
-      for (int i = 0; i < 16; ++i) {
-        const uint32_t e = celem<LH, 0, 4>(t, 0, i);
-        uint64_t v = lds[pidx(e)];
-        if constexpr (MODE == MODE_NINV) v = gl::mul(v, a.n_inv);
-        if constexpr (MODE == MODE_LDE_OUT) v = gl::mul(v, contig_scale<MODE, DIT>(a, ct.pos(e)));
-        bool ok;
-        const uint64_t ad = ct.addr(e, a.stride_out, ct.base_out, a.num_cols, ok);
-        if (ok) a.out[ad] = v;  // canonical: the field ops keep values in [0, p)
-      }
-    } else {
-      // regroup the coalesced load into the last level's runs through LDS
-#pragma unroll
-      for (int i = 0; i < 16; ++i) lds[pidx(celem<LH, 0, 4>(t, 0, i))] = x[i];
-      __syncthreads();
-#pragma unroll
-      for (int g = 0; g < GL; ++g)
-#pragma unroll
-        for (int i = 0; i < (1 << AL); ++i) x[g * (1 << AL) + i] = lds[pidx(celem<LH, LL, AL>(t, g, i))];
-      if constexpr (NL == 3) {
-#pragma unroll
-        for (int g = 0; g < G2; ++g) dft_dit<A2, INV>(x + g * (1 << A2));
-#pragma unroll
-        for (int g = 0; g < G2; ++g)
-#pragma unroll
-          for (int i = 0; i < (1 << A2); ++i) lds[pidx(celem<LH, 2, A2>(t, g, i))] = x[g * (1 << A2) + i];
-        __syncthreads();
-#pragma unroll
-        for (int i = 0; i < 16; ++i) x[i] = lds[pidx(celem<LH, 1, 4>(t, 0, i))];
-        level_twist_lds<LH - 4>(x, tw1, rp1);
-      }
-      dft_dit<4, INV>(x);
-#pragma unroll
-      for (int i = 0; i < 16; ++i) lds[pidx(celem<LH, 1, 4>(t, 0, i))] = x[i];
-      __syncthreads();
-#pragma unroll
-      for (int i = 0; i < 16; ++i) x[i] = lds[pidx(celem<LH, 0, 4>(t, 0, i))];
-#pragma unroll
-      for (int i = 1; i < 16; ++i) x[i] = gl::mul(x[i], tw0[i - 1]);
-      dft_dit<4, INV>(x);
-#pragma unroll
-      for (int i = 0; i < 16; ++i) {
-        const uint32_t e = celem<LH, 0, 4>(t, 0, i);
-        uint64_t v = x[i];
-        if constexpr (MODE == MODE_NINV) v = gl::mul(v, a.n_inv);
-        if constexpr (MODE == MODE_LDE_OUT) v = gl::mul(v, contig_scale<MODE, DIT>(a, ct.pos(e)));
-        bool ok;
-        const uint64_t ad = ct.addr(e, a.stride_out, ct.base_out, a.num_cols, ok);
-        if (ok) a.out[ad] = v;
-      }
-    }
-    __syncthreads();  // LDS is reused by the next tile
-  }
+  __shared__ PassLds sh;
+  contig_run<LH, INV, DIT, MODE, MC>(a, sh, blockIdx.x, a.tiles, gridDim.x);
 }
-
-// -------------------------------------------------------------------------
-// strided<h>
 
 template <int LH, bool INV, bool DIT, int MODE>
 __global__ __launch_bounds__(NT) void ntt_strided_kernel(FastArgs a) {
-  constexpr int LT = 12 - LH;         // log2 sub-DFTs (consecutive r) per tile
-  constexpr int T = 1 << LT;
-  constexpr int NL = LH > 4 ? 2 : 1;
-  constexpr int A1 = NL == 2 ? LH - 4 : 4;  // second level width
-  constexpr int G1 = 16 >> A1;
-  constexpr int LS0 = LH - 4;               // level-0 stride (log2)
-  constexpr uint32_t TW0_LEN = NL == 2 ? (15u << (LH - 4)) : 1u;
-  __shared__ uint64_t lds[NL == 2 ? TILE : 1];
-  __shared__ uint64_t tw0[TW0_LEN];
-  const uint32_t t = threadIdx.x;
-  const uint32_t log_r = a.log_N - LH;
-  const uint32_t r0 = blockIdx.x << LT;
-
-  // level-0 mapping: q = t % T, rp = t / T, j = i * 2^LS0 + rp
-  const uint32_t q0 = t & (T - 1u), rp0 = t >> LT;
-  // last-level mapping (level 1, or level 0 when h = 4): q = gi % T, b = gi / T, j = b * 2^A1 + i
-  auto lastj = [&](int g, int i) -> uint32_t {
-    if constexpr (NL == 2) return (((t + NT * (uint32_t)g) >> LT) << A1) + (uint32_t)i;
-    else return ((uint32_t)i << LS0) + rp0;
-  };
-  auto lastq = [&](int g) -> uint32_t {
-    if constexpr (NL == 2) return (t + NT * (uint32_t)g) & (T - 1u);
-    else return q0;
-  };
-  // Element offsets from the block base (r0 included).  Level-0 mapping:
-  // base0 + i * step0; last-level mapping: basel(g) + i * stepl.
-  const uint64_t step0 = 1ull << (LS0 + log_r);
-  const uint64_t base0 = ((uint64_t)rp0 << log_r) + q0;
-  const uint64_t stepl = NL == 2 ? (1ull << log_r) : step0;
-  auto basel = [&](int g) -> uint64_t {
-    if constexpr (NL == 2) return ((uint64_t)lastj(g, 0) << log_r) + lastq(g);
-    else return base0;
-  };
-
-  if constexpr (NL == 2) stage_level_twists<LH, INV>(tw0, a.tab);
-
-  // pass twist omega_N^((r0 + q) * brev_h(j)) in the last-level mapping, with
-  // n^-1 folded in when this pass carries the inverse scaling
-  uint64_t pt[16];
-#pragma unroll
-  for (int g = 0; g < G1; ++g)
-#pragma unroll
-    for (int i = 0; i < (1 << A1); ++i) {
-      const uint32_t j = lastj(g, i);
-      uint32_t e = ((r0 + lastq(g)) * brev_bits(j, LH)) << (32u - a.log_N);
-      if (INV) e = 0u - e;
-      uint64_t w = pow_w(a.tab, e);
-      if constexpr (MODE == MODE_NINV) w = gl::mul(w, a.n_inv);
-      pt[g * (1 << A1) + i] = w;
-    }
-  // coset factors at column positions j * R + r0 + q (this is the top pass:
-  // one block per column), level-0 mapping
-  uint64_t cf[MODE == MODE_LDE_IN || MODE == MODE_LDE_OUT ? 16 : 1];
-  if constexpr (MODE == MODE_LDE_IN || MODE == MODE_LDE_OUT) {
-#pragma unroll
-    for (int i = 0; i < 16; ++i) {
-      const uint32_t pos = ((((uint32_t)i << LS0) + rp0) << log_r) + r0 + q0;
-      if constexpr (MODE == MODE_LDE_IN) {
-        cf[i] = coset_pow(a.tab, a.coset_exp, pos, false);  // n2b input: natural index
-      } else {
-        cf[i] = gl::mul(a.n_inv, coset_pow(a.tab, a.coset_exp, pos, true));  // b2n output: natural index
-      }
-    }
-  }
-
-  const uint32_t blocks_per_col_log = a.log_n - a.log_N;
-  const uint64_t total = (uint64_t)a.num_cols << blocks_per_col_log;
+  __shared__ PassLds sh;
+  const uint64_t total = (uint64_t)a.num_cols << (a.log_n - a.log_N);
   const uint64_t p_begin = (uint64_t)blockIdx.y * a.blocks_per_wg;
   const uint64_t p_end = p_begin + a.blocks_per_wg < total ? p_begin + a.blocks_per_wg : total;
-  auto pair_off = [&](uint64_t p, uint64_t stride) -> uint64_t {
-    const uint64_t col = p >> blocks_per_col_log;
-    const uint64_t blk = p & ((1ull << blocks_per_col_log) - 1ull);
-    return col * stride + (blk << a.log_N) + r0;
-  };
-  // loads one pair's 16 values in the first level's mapping
-  auto load = [&](uint64_t p, uint64_t *v) {
-    const uint64_t *in = a.in + pair_off(p, a.stride_in);
-    if constexpr (!DIT) {
-      const uint64_t *q = in + base0;
-#pragma unroll
-      for (int k = 0; k < 16; ++k, q += step0) v[k] = *q;
-    } else {
-#pragma unroll
-      for (int g = 0; g < G1; ++g) {
-        const uint64_t *q = in + basel(g);
-#pragma unroll
-        for (int i = 0; i < (1 << A1); ++i, q += stepl) v[g * (1 << A1) + i] = *q;
-      }
-    }
-  };
-
-  uint64_t pf[16];
-  if (p_begin < p_end) load(p_begin, pf);
-  if constexpr (NL == 2) __syncthreads();  // tw0 staged
-  for (uint64_t p = p_begin; p < p_end; ++p) {
-    uint64_t x[16];
-#pragma unroll
-    for (int i = 0; i < 16; ++i) x[i] = pf[i];
-    if (p + 1 < p_end) load(p + 1, pf);  // prefetch: in flight during this pair's math
-    uint64_t *out = a.out + pair_off(p, a.stride_out);
-    if constexpr (!DIT) {
-      if constexpr (MODE == MODE_LDE_IN) {
-#pragma unroll
-        for (int i = 0; i < 16; ++i) x[i] = gl::mul(x[i], cf[i]);
-      } else if (a.first) {
-#pragma unroll
-        for (int i = 0; i < 16; ++i) x[i] = gl::canon(x[i]);  // field ops take canonical values
-      }
-      dft_dif<4, INV>(x);
-      if constexpr (NL == 2) {
-        level_twist_lds<LH>(x, tw0, rp0);
-        // LDS row-major by j: element (q, j) at j * T + q
-#pragma unroll
-        for (int i = 0; i < 16; ++i) lds[((((uint32_t)i << LS0) + rp0) << LT) + q0] = x[i];
-        __syncthreads();
-#pragma unroll
-        for (int g = 0; g < G1; ++g)
-#pragma unroll
-          for (int i = 0; i < (1 << A1); ++i) x[g * (1 << A1) + i] = lds[(lastj(g, i) << LT) + lastq(g)];
-        __syncthreads();  // the next iteration rewrites LDS
-#pragma unroll
-        for (int g = 0; g < G1; ++g) dft_dif<A1, INV>(x + g * (1 << A1));
-      }
-#pragma unroll
-      for (int g = 0; g < G1; ++g) {
-        uint64_t *q = out + basel(g);
-#pragma unroll
-        for (int i = 0; i < (1 << A1); ++i, q += stepl) {
-          const int k = g * (1 << A1) + i;
-          *q = gl::mul(x[k], pt[k]);
-        }
-      }
-    } else {
-      // the pass twist makes every value canonical
-#pragma unroll
-      for (int k = 0; k < 16; ++k) x[k] = gl::mul(x[k], pt[k]);
-      if constexpr (NL == 2) {
-#pragma unroll
-        for (int g = 0; g < G1; ++g) dft_dit<A1, INV>(x + g * (1 << A1));
-#pragma unroll
-        for (int g = 0; g < G1; ++g)
-#pragma unroll
-          for (int i = 0; i < (1 << A1); ++i) lds[(lastj(g, i) << LT) + lastq(g)] = x[g * (1 << A1) + i];
-        __syncthreads();
-#pragma unroll
-        for (int i = 0; i < 16; ++i) x[i] = lds[((((uint32_t)i << LS0) + rp0) << LT) + q0];
-        __syncthreads();
-        level_twist_lds<LH>(x, tw0, rp0);
-      }
-      dft_dit<4, INV>(x);
-      uint64_t *q = out + base0;
-#pragma unroll
-      for (int i = 0; i < 16; ++i, q += step0) {
-        uint64_t v = x[i];
-        if constexpr (MODE == MODE_LDE_OUT) v = gl::mul(v, cf[i]);
-        *q = v;
-      }
-    }
-  }
+  strided_run<LH, INV, DIT, MODE>(a, sh, blockIdx.x, p_begin, p_end);
 }
 
 // -------------------------------------------------------------------------

@@ -85,6 +85,23 @@ enum PassMode : int {
   MODE_LDE_OUT = 3,  // multiply outputs by n^-1 s^-i (inverse coset transform)
 };
 
+// One pass of a planned transform (log_n >= 8), as launched by
+// bc_batch_ntt_internal and scheduled by the ragged persistent kernel.
+struct PassDesc {
+  bool contig;         // contiguous pass (else strided)
+  uint32_t log_h;      // sub-transform size of the pass
+  int mode;            // PassMode
+  FastArgs a;          // in/out/strides/flags; a.tiles for contiguous passes
+  uint64_t positions;  // strided: number of r-ranges (4096-point tiles per block)
+  uint64_t pairs;      // strided: (column, block) pairs
+};
+
+// Plans a transform into passes in execution order; returns the pass count
+// (at most 8).  Arguments must already be validated; requires log_n >= 8.
+int describe_fast_passes(const Tables &tab, const uint64_t *inputs, uint64_t *outputs, uint32_t log_n,
+                         uint32_t num_cols, uint64_t stride_in, uint64_t stride_out, bool bitrev_inputs, bool inverse,
+                         uint32_t log_ext, uint32_t coset_index, PassDesc *out);
+
 hipError_t launch_contig(const FastArgs &a, int log_h, bool inverse, bool dit, int mode, hipStream_t stream);
 hipError_t launch_strided(const FastArgs &a, int log_h, bool inverse, bool dit, int mode, uint32_t positions,
                           uint32_t groups, hipStream_t stream);

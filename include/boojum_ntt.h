@@ -105,6 +105,52 @@ int bc_lde_graph_create(const uint64_t *inputs, uint64_t *coeffs, uint64_t *outp
 int bc_graph_launch(bc_graph *graph, void *stream);
 int bc_graph_destroy(bc_graph *graph);
 
+/* Ragged batches: many independent jobs of different sizes and operations,
+ * run by one persistent kernel that pulls work units from a device queue.
+ * Each job transforms num_cols columns of 2^log_n points (8 <= log_n):
+ *   BC_OP_NTT   forward, natural -> bit-reversed (bc_batch_ntt_internal with
+ *               bitrev_inputs = false, inverse = false)
+ *   BC_OP_INTT  inverse, bit-reversed -> natural (bitrev_inputs = true,
+ *               inverse = true)
+ *   BC_OP_LDE   bit-reversed evaluations -> coefficients (written to coeffs)
+ *               -> bit-reversed LDE with blowup 2^log_blowup (bc_batch_lde
+ *               with bitrev_inputs = true)
+ * No reference counterpart: the reference launches each job separately. */
+#define BC_OP_NTT 0
+#define BC_OP_INTT 1
+#define BC_OP_LDE 2
+
+typedef struct {
+  const uint64_t *inputs; /* column i at inputs + i * stride_in */
+  uint64_t *outputs;      /* column i at outputs + i * stride_out */
+  uint64_t *coeffs;       /* BC_OP_LDE only: column i at coeffs + i * stride_coeffs */
+  uint32_t log_n;
+  uint32_t num_cols;
+  uint32_t stride_in;
+  uint32_t stride_out;    /* >= n, or n << log_blowup for BC_OP_LDE */
+  uint32_t stride_coeffs;
+  uint32_t op;
+  uint32_t log_blowup;    /* BC_OP_LDE only */
+} bc_ragged_job;
+
+typedef struct bc_ragged_plan bc_ragged_plan;
+
+/* Plans the jobs into work units (host + one device upload).  The plan keeps
+ * the job pointers; buffers must stay valid while it is run. */
+int bc_ragged_plan_create(const bc_ragged_job *jobs, uint32_t num_jobs, bc_ragged_plan **out);
+/* Resets the queue and runs every job with one persistent launch on stream. */
+int bc_ragged_run(bc_ragged_plan *plan, void *stream);
+/* After the stream has completed a run: *failed = 1 if a work unit gave up
+ * waiting for its predecessor (a bounded wait that never expires in a
+ * correct plan); the outputs are then not valid. */
+int bc_ragged_status(bc_ragged_plan *plan, int *failed);
+int bc_ragged_plan_destroy(bc_ragged_plan *plan);
+/* Statistics of a plan: work units, dependency levels, persistent workgroups. */
+int bc_ragged_plan_info(const bc_ragged_plan *plan, uint64_t *units, uint32_t *levels, uint32_t *workgroups);
+/* The plain path for comparison: the same jobs as separate launch sequences
+ * (one bc_batch_ntt_internal / bc_batch_lde per job). */
+int bc_ragged_run_plain(const bc_ragged_job *jobs, uint32_t num_jobs, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
