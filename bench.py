@@ -73,6 +73,130 @@ def bench_lde(log_n, cols, log_blowup, reps, warmup):
                          "algorithmic_bytes_per_call": algo}}
 
 
+def ragged_specs(count, log_lo, log_hi, seed):
+    """Single-column jobs: log_n uniform in [log_lo, log_hi], op uniform over
+    NTT, iNTT, LDE blowup 2, LDE blowup 4."""
+    import random
+    from boojum_cuda_amd import ragged
+    rng = random.Random(seed)
+    kinds = [(ragged.OP_NTT, 0), (ragged.OP_INTT, 0), (ragged.OP_LDE, 1), (ragged.OP_LDE, 2)]
+    return [(rng.randint(log_lo, log_hi),) + kinds[rng.randrange(4)] for _ in range(count)]
+
+
+def bench_ragged(count, log_lo, log_hi, reps, warmup):
+    """Ragged batch: one persistent launch over every job vs the plain path
+    (separate launch sequences per job, on the same stream).  Both must give
+    the same bits.  Algorithmic bytes: 16 n (NTT/iNTT), 8 (1 + B) n (LDE)."""
+    import torch
+    from boojum_cuda_amd import ragged
+    specs = ragged_specs(count, log_lo, log_hi, 4)
+    g = torch.Generator(device="cuda")
+    g.manual_seed(4)
+    jobs, algo, out_elems, in_elems = [], 0.0, 0, 0
+    for log_n, op, lb in specs:
+        n = 1 << log_n
+        x = uniform_field(n, g)
+        out = torch.empty(n << lb, dtype=torch.int64, device="cuda")
+        j = ragged.Job(op=op, log_n=log_n, inputs=x, outputs=out, log_blowup=lb)
+        if op == ragged.OP_LDE:
+            j.coeffs = torch.empty(n, dtype=torch.int64, device="cuda")
+        jobs.append(j)
+        algo += 8.0 * (n + (n << lb))
+        out_elems += n << lb
+        in_elems += n
+    plan = ragged.RaggedPlan(jobs)
+    info = plan.info()
+    stream = torch.cuda.current_stream()
+
+    def timed(fn):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2 * reps)]
+        for r in range(reps):
+            ev[2 * r].record(stream)
+            fn()
+            ev[2 * r + 1].record(stream)
+        torch.cuda.synchronize()
+        return sorted(ev[2 * r].elapsed_time(ev[2 * r + 1]) for r in range(reps))[reps // 2]
+
+    ms_ragged = timed(plan.run)
+    failed = plan.failed()
+    ragged_out = [j.outputs.clone() for j in jobs]
+    ms_plain = timed(lambda: ragged.run_plain(jobs))
+    same = all(torch.equal(a, j.outputs) for a, j in zip(ragged_out, jobs))
+    plan.destroy()
+    del ragged_out, jobs
+    torch.cuda.empty_cache()
+    gbs = algo / (ms_ragged * 1e-3) / 1e9
+    return {"workload": f"ragged batch: {count} single-column jobs, log_n ~ U[{log_lo},{log_hi}], "
+                        f"op ~ U{{NTT, iNTT, LDE x2, LDE x4}}, one persistent launch",
+            "ms_ragged": ms_ragged, "ms_plain": ms_plain, "speedup_vs_plain": ms_plain / ms_ragged,
+            "input_elems": in_elems, "output_elems": out_elems,
+            "output_elems_per_s": out_elems / (ms_ragged * 1e-3),
+            "units": info["units"], "levels": info["levels"], "workgroups": info["workgroups"],
+            "bit_identical_to_plain": bool(same and not failed),
+            "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": gbs / HBM_PEAK_GBS, "algorithmic_bytes_per_call": algo}}
+
+
+def bench_serve(count, rate, budget_bytes, seed=5):
+    """Serving loop at a fixed offered load: `count` requests with Poisson
+    arrivals at `rate` per second, each 1-512 columns of 2^log_n seeded
+    values, log_n ~ U[12,24], blowup ~ U{1,2,4} (1: forward NTT, else coset
+    LDE), admitted under an HBM budget of `budget_bytes`.  Latency = submit to
+    completion (host clock); throughput = output elements over the makespan."""
+    import random
+    import numpy as np
+    from boojum_cuda_amd import serve
+    rng = random.Random(seed)
+    specs, t = [], 0.0
+    for _ in range(count):
+        t += rng.expovariate(rate)
+        lb = rng.choice([0, 1, 2])
+        specs.append((t, rng.randint(12, 24), rng.randint(1, 512), lb))
+    srv = serve.Server(budget_bytes)
+    try:
+        warm = [srv.submit(serve.OP_LDE, 12, 4, lb, seed=1) for lb in (1, 2)] + [srv.submit(serve.OP_NTT, 16, 2)]
+        srv.drain()
+        for rid in warm:
+            srv.release(rid)
+        ids, out_elems, algo = [], 0, 0.0
+        t0 = time.perf_counter()
+        for k, (at, log_n, cols, lb) in enumerate(specs):
+            while True:
+                dt = t0 + at - time.perf_counter()
+                if dt <= 0:
+                    break
+                time.sleep(min(dt, 0.0005))
+            op = serve.OP_LDE if lb else serve.OP_NTT
+            ids.append(srv.submit(op, log_n, cols, lb, seed=1000 + k))
+            n = 1 << log_n
+            out_elems += (n << lb) * cols
+            algo += 8.0 * (n + (n << lb)) * cols
+        srv.drain()
+        infos = [srv.info(rid) for rid in ids]
+        st = srv.stats()
+        for rid in ids:
+            srv.release(rid)
+    finally:
+        srv.destroy()
+    assert all(i["status"] == 1 for i in infos), "a request failed"
+    lat_ms = np.array([(i["t_done_ns"] - i["t_submit_ns"]) * 1e-6 for i in infos])
+    span_s = (max(i["t_done_ns"] for i in infos) - min(i["t_submit_ns"] for i in infos)) * 1e-9
+    gbs = algo / span_s / 1e9
+    return {"workload": f"serving: {count} requests, Poisson arrivals at {rate}/s, cols ~ U[1,512], "
+                        f"log_n ~ U[12,24], blowup ~ U{{1,2,4}}, seeded inputs, {budget_bytes / 1e9:.0f} GB budget",
+            "offered_load_req_per_s": rate, "makespan_s": span_s,
+            "p50_latency_ms": float(np.percentile(lat_ms, 50)), "p99_latency_ms": float(np.percentile(lat_ms, 99)),
+            "mean_latency_ms": float(lat_ms.mean()), "output_elems_per_s": out_elems / span_s,
+            "requests_per_s": count / span_s, "parts": st["parts_done"], "batches": st["batches_launched"],
+            "max_batches_in_flight_seen": st["max_batches_in_flight_seen"],
+            "arena_high_water_gb": st["arena_high_water_bytes"] / 1e9,
+            "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": gbs / HBM_PEAK_GBS}}
+
+
 def cpu_baseline(log_n, cols, threads):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import numpy as np
@@ -87,6 +211,10 @@ def cpu_baseline(log_n, cols, threads):
             "sample": f"{cols} columns x 2^{log_n}, forward n2b, {dt:.2f} s wall"}
 
 
+def progress(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -97,6 +225,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-cols", type=int, default=64)
     ap.add_argument("--no-lde", action="store_true", help="skip the coset-LDE measurement")
+    ap.add_argument("--no-ragged", action="store_true", help="skip the ragged-batch measurement")
+    ap.add_argument("--ragged-jobs", type=int, default=1000)
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop measurement")
+    ap.add_argument("--serve-requests", type=int, default=100)
+    ap.add_argument("--serve-rate", type=float, default=25.0, help="offered load, requests per second")
+    ap.add_argument("--serve-budget-gb", type=float, default=200.0)
     ap.add_argument("--mode", choices=["ntt", "copy"], default="ntt",
                     help="copy: time a plain device copy of the same matrix (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -161,9 +295,20 @@ def main():
     algo_bytes = 16.0 * cols * n
     achieved = algo_bytes / (avg_call_ms * 1e-3) / 1e9
     del x, y
+    progress(f"batched NTT done: {avg_call_ms:.3f} ms per call")
     extras = {}
     if not args.no_lde:
         extras["lde"] = bench_lde(20, 128, 2, max(3, args.steps // 2), 2)
+        torch.cuda.empty_cache()
+        progress(f"coset LDE done: {extras['lde']['ms_per_call']:.3f} ms per call")
+    if not args.no_ragged:
+        extras["ragged"] = bench_ragged(args.ragged_jobs, 10, 23, 5, 1)
+        progress(f"ragged done: {extras['ragged']['ms_ragged']:.2f} ms vs plain {extras['ragged']['ms_plain']:.2f} ms")
+    if not args.no_serve:
+        torch.cuda.empty_cache()
+        extras["serve"] = bench_serve(args.serve_requests, args.serve_rate, int(args.serve_budget_gb * 1e9))
+        progress(f"serving done: p50 {extras['serve']['p50_latency_ms']:.1f} ms, "
+                 f"p99 {extras['serve']['p99_latency_ms']:.1f} ms")
     ctx.destroy()
 
     rec = {
@@ -189,6 +334,7 @@ def main():
     rec.update(extras)
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
+        progress(f"cpu baseline on {threads} threads")
         rec["cpu_baseline"] = cpu_baseline(log_n, args.cpu_cols, threads)
     print(json.dumps(rec), flush=True)
 

@@ -34,6 +34,16 @@ SIGNATURES = {
     "bc_ragged_plan_destroy": (c_int, [c_vp]),
     "bc_ragged_plan_info": (c_int, [c_vp, ctypes.POINTER(c_u64), ctypes.POINTER(c_u32), ctypes.POINTER(c_u32)]),
     "bc_ragged_run_plain": (c_int, [c_vp, c_u32, c_vp]),
+    "bc_fill_seeded": (c_int, [c_vp, c_u32, c_u32, c_u64, c_u64, c_u32, c_vp]),
+    "bc_server_create": (c_int, [c_vp, ctypes.POINTER(c_vp)]),
+    "bc_server_submit": (c_int, [c_vp, c_vp, ctypes.POINTER(c_u64)]),
+    "bc_server_wait": (c_int, [c_vp, c_u64]),
+    "bc_server_request_info": (c_int, [c_vp, c_u64, c_vp]),
+    "bc_server_part": (c_int, [c_vp, c_u64, c_u32, c_vp]),
+    "bc_server_release": (c_int, [c_vp, c_u64]),
+    "bc_server_drain": (c_int, [c_vp]),
+    "bc_server_stats_get": (c_int, [c_vp, c_vp]),
+    "bc_server_destroy": (c_int, [c_vp]),
 }
 
 BC_SUCCESS = 0

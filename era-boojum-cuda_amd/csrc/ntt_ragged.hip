@@ -16,26 +16,13 @@
 // device-scope release and bump the segment's completion counter.  Waiting
 // only ever targets units already taken by running workgroups, so the queue
 // always drains; the bound turns a planning error into a flag, not a hang.
-#include "../../include/boojum_ntt.h"
+#include "ntt_ragged.hpp"
 #include "ntt_passes.hpp"
 
 #include <algorithm>
-#include <mutex>
 #include <vector>
 
 namespace bc {
-
-struct RaggedSeg {
-  FastArgs a;
-  uint64_t first_item;  // queue position of the first unit
-  uint64_t total;       // contiguous: tiles; strided: (column, block) pairs
-  uint32_t units;
-  uint32_t variant;
-  int32_t dep;          // segment that must be complete first, -1 if none
-  uint32_t dep_units;
-  uint32_t per_unit;    // contiguous: tiles per unit; strided: pairs per unit
-  uint32_t positions;   // strided: r-ranges
-};
 
 namespace {
 using namespace pass;
@@ -200,31 +187,17 @@ uint32_t brev_bits_host(uint32_t x, uint32_t bits) {
   return r;
 }
 
-}  // namespace
-
-Tables current_tables(bool &ok);  // ntt_api.hip
-
-}  // namespace bc
-
-struct bc_ragged_plan {
-  int device;
-  uint32_t nsegs;
-  uint64_t total_items;
-  uint32_t levels;
-  uint32_t workgroups;
-  bc::RaggedSeg *d_segs;
-  bc::QueueState *d_state;
-};
-
-namespace {
-
 struct PlanSeg {
-  bc::RaggedSeg s;
+  RaggedSeg s;
   uint32_t level, job, stage, pass;
   int32_t dep_key;  // index into the unsorted list
 };
 
-int validate_job(const bc_ragged_job &j) {
+}  // namespace
+
+Tables current_tables(bool &ok);  // ntt_api.hip
+
+int ragged_validate_job(const bc_ragged_job &j) {
   if (j.log_n < 8 || j.log_n >= 32 || !j.inputs || !j.outputs) return BC_ERROR_INVALID_VALUE;
   const uint64_t n = 1ull << j.log_n;
   if (j.stride_in < n) return BC_ERROR_INVALID_VALUE;
@@ -232,7 +205,7 @@ int validate_job(const bc_ragged_job &j) {
     if (j.stride_out < n) return BC_ERROR_INVALID_VALUE;
   } else if (j.op == BC_OP_LDE) {
     if (!j.coeffs || j.stride_coeffs < n || j.log_blowup < 1 || j.log_blowup > 8) return BC_ERROR_INVALID_VALUE;
-    if ((uint64_t)j.log_n + j.log_blowup > bc::OMEGA_LOG_ORDER) return BC_ERROR_INVALID_VALUE;
+    if ((uint64_t)j.log_n + j.log_blowup > OMEGA_LOG_ORDER) return BC_ERROR_INVALID_VALUE;
     if ((uint64_t)j.stride_out < (n << j.log_blowup)) return BC_ERROR_INVALID_VALUE;
   } else {
     return BC_ERROR_INVALID_VALUE;
@@ -240,20 +213,14 @@ int validate_job(const bc_ragged_job &j) {
   return BC_SUCCESS;
 }
 
-}  // namespace
-
-extern "C" int bc_ragged_plan_create(const bc_ragged_job *jobs, uint32_t num_jobs, bc_ragged_plan **out) {
-  using namespace bc;
-  if (!out || (!jobs && num_jobs)) return BC_ERROR_INVALID_VALUE;
-  *out = nullptr;
+int ragged_build(const bc_ragged_job *jobs, uint32_t num_jobs, const Tables &tab, RaggedHostPlan &out) {
+  out.segs.clear();
+  out.total_items = 0;
+  out.levels = 0;
   for (uint32_t j = 0; j < num_jobs; ++j) {
-    const int rc = validate_job(jobs[j]);
+    const int rc = ragged_validate_job(jobs[j]);
     if (rc != BC_SUCCESS) return rc;
   }
-  bool ok = false;
-  const Tables tab = current_tables(ok);
-  if (!ok) return BC_ERROR_NOT_INITIALIZED;
-
   std::vector<PlanSeg> segs;
   for (uint32_t j = 0; j < num_jobs; ++j) {
     const bc_ragged_job &jb = jobs[j];
@@ -335,9 +302,8 @@ extern "C" int bc_ragged_plan_create(const bc_ragged_job *jobs, uint32_t num_job
   });
   std::vector<uint32_t> pos_of(segs.size());
   for (uint32_t i = 0; i < order.size(); ++i) pos_of[order[i]] = i;
-  std::vector<RaggedSeg> host(segs.size());
+  out.segs.resize(segs.size());
   uint64_t item = 0;
-  uint32_t levels = 0;
   for (uint32_t i = 0; i < order.size(); ++i) {
     const PlanSeg &ps = segs[order[i]];
     RaggedSeg r = ps.s;
@@ -351,29 +317,85 @@ extern "C" int bc_ragged_plan_create(const bc_ragged_job *jobs, uint32_t num_job
       r.dep = -1;
       r.dep_units = 0;
     }
-    levels = std::max(levels, ps.level + 1);
-    host[i] = r;
+    out.levels = std::max(out.levels, ps.level + 1);
+    out.segs[i] = r;
   }
+  out.total_items = item;
+  return BC_SUCCESS;
+}
+
+size_t ragged_state_bytes(uint32_t nsegs) {
+  return sizeof(QueueState) + std::max<size_t>(1, nsegs) * sizeof(uint32_t);
+}
+
+uint32_t ragged_workgroups(int dev) {
+  int cus = 0, per_cu = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) cus = 256;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, ntt_ragged_kernel, NT, 0) != hipSuccess || per_cu <= 0)
+    per_cu = 1;
+  return (uint32_t)(cus * per_cu);
+}
+
+hipError_t ragged_launch(const RaggedSeg *d_segs, uint32_t nsegs, uint64_t total_items, void *d_state,
+                         uint32_t workgroups, hipStream_t s) {
+  QueueState *st = (QueueState *)d_state;
+  ragged_reset_kernel<<<dim3((nsegs + 255) / 256 + 1), dim3(256), 0, s>>>(st, nsegs);
+  hipError_t err = hipGetLastError();
+  if (err != hipSuccess) return err;
+  if (total_items == 0) return hipSuccess;
+  const uint32_t wgs = (uint32_t)std::min<uint64_t>(std::max(1u, workgroups), total_items);
+  // ~0.2 us per sleep: the bound is seconds, far beyond any real wait
+  ntt_ragged_kernel<<<dim3(wgs), dim3(NT), 0, s>>>(d_segs, nsegs, total_items, st, 1u << 24);
+  return hipGetLastError();
+}
+
+uint32_t *ragged_error_ptr(void *d_state) { return &((QueueState *)d_state)->error; }
+
+hipError_t ragged_read_error(const void *d_state, uint32_t *error) {
+  return hipMemcpy(error, &((const QueueState *)d_state)->error, sizeof(uint32_t), hipMemcpyDeviceToHost);
+}
+
+}  // namespace bc
+
+struct bc_ragged_plan {
+  int device;
+  uint32_t nsegs;
+  uint64_t total_items;
+  uint32_t levels;
+  uint32_t workgroups;
+  bc::RaggedSeg *d_segs;
+  void *d_state;
+};
+
+extern "C" int bc_ragged_plan_create(const bc_ragged_job *jobs, uint32_t num_jobs, bc_ragged_plan **out) {
+  using namespace bc;
+  if (!out || (!jobs && num_jobs)) return BC_ERROR_INVALID_VALUE;
+  *out = nullptr;
+  for (uint32_t j = 0; j < num_jobs; ++j) {
+    const int rc = ragged_validate_job(jobs[j]);
+    if (rc != BC_SUCCESS) return rc;
+  }
+  bool ok = false;
+  const Tables tab = current_tables(ok);
+  if (!ok) return BC_ERROR_NOT_INITIALIZED;
+  RaggedHostPlan hp;
+  int rc = ragged_build(jobs, num_jobs, tab, hp);
+  if (rc != BC_SUCCESS) return rc;
 
   int dev = 0;
   hipError_t err = hipGetDevice(&dev);
   if (err != hipSuccess) return err;
   bc_ragged_plan *p = new bc_ragged_plan{};
   p->device = dev;
-  p->nsegs = (uint32_t)host.size();
-  p->total_items = item;
-  p->levels = levels;
-  int cus = 0, per_cu = 0;
-  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) cus = 256;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, ntt_ragged_kernel, NT, 0) != hipSuccess || per_cu <= 0)
-    per_cu = 1;
-  p->workgroups = (uint32_t)(cus * per_cu);
-  const size_t seg_bytes = std::max<size_t>(1, host.size()) * sizeof(RaggedSeg);
-  const size_t st_bytes = sizeof(QueueState) + std::max<size_t>(1, host.size()) * sizeof(uint32_t);
+  p->nsegs = (uint32_t)hp.segs.size();
+  p->total_items = hp.total_items;
+  p->levels = hp.levels;
+  p->workgroups = ragged_workgroups(dev);
+  const size_t seg_bytes = std::max<size_t>(1, hp.segs.size()) * sizeof(RaggedSeg);
   err = hipMalloc((void **)&p->d_segs, seg_bytes);
-  if (err == hipSuccess) err = hipMalloc((void **)&p->d_state, st_bytes);
-  if (err == hipSuccess && !host.empty())
-    err = hipMemcpy(p->d_segs, host.data(), host.size() * sizeof(RaggedSeg), hipMemcpyHostToDevice);
+  if (err == hipSuccess) err = hipMalloc(&p->d_state, ragged_state_bytes(p->nsegs));
+  if (err == hipSuccess && !hp.segs.empty())
+    err = hipMemcpy(p->d_segs, hp.segs.data(), hp.segs.size() * sizeof(RaggedSeg), hipMemcpyHostToDevice);
   if (err != hipSuccess) {
     (void)hipFree(p->d_segs);
     (void)hipFree(p->d_state);
@@ -385,23 +407,14 @@ extern "C" int bc_ragged_plan_create(const bc_ragged_job *jobs, uint32_t num_job
 }
 
 extern "C" int bc_ragged_run(bc_ragged_plan *p, void *stream) {
-  using namespace bc;
   if (!p) return BC_ERROR_INVALID_VALUE;
-  hipStream_t s = (hipStream_t)stream;
-  ragged_reset_kernel<<<dim3((p->nsegs + 255) / 256 + 1), dim3(256), 0, s>>>(p->d_state, p->nsegs);
-  hipError_t err = hipGetLastError();
-  if (err != hipSuccess) return err;
-  if (p->total_items == 0) return BC_SUCCESS;
-  const uint32_t wgs = (uint32_t)std::min<uint64_t>(p->workgroups, p->total_items);
-  // ~0.2 us per sleep: the bound is seconds, far beyond any real wait
-  ntt_ragged_kernel<<<dim3(wgs), dim3(NT), 0, s>>>(p->d_segs, p->nsegs, p->total_items, p->d_state, 1u << 24);
-  return hipGetLastError();
+  return bc::ragged_launch(p->d_segs, p->nsegs, p->total_items, p->d_state, p->workgroups, (hipStream_t)stream);
 }
 
 extern "C" int bc_ragged_status(bc_ragged_plan *p, int *failed) {
   if (!p || !failed) return BC_ERROR_INVALID_VALUE;
   uint32_t e = 0;
-  hipError_t err = hipMemcpy(&e, &p->d_state->error, sizeof(e), hipMemcpyDeviceToHost);
+  hipError_t err = bc::ragged_read_error(p->d_state, &e);
   if (err != hipSuccess) return err;
   *failed = e ? 1 : 0;
   return BC_SUCCESS;
@@ -427,7 +440,7 @@ extern "C" int bc_ragged_run_plain(const bc_ragged_job *jobs, uint32_t num_jobs,
   if (!jobs && num_jobs) return BC_ERROR_INVALID_VALUE;
   for (uint32_t j = 0; j < num_jobs; ++j) {
     const bc_ragged_job &jb = jobs[j];
-    int rc = validate_job(jb);
+    int rc = bc::ragged_validate_job(jb);
     if (rc != BC_SUCCESS) return rc;
     if (jb.op == BC_OP_NTT)
       rc = bc_batch_ntt_internal(jb.inputs, jb.outputs, jb.log_n, jb.num_cols, jb.stride_in, jb.stride_out, false,

@@ -151,6 +151,93 @@ int bc_ragged_plan_info(const bc_ragged_plan *plan, uint64_t *units, uint32_t *l
  * (one bc_batch_ntt_internal / bc_batch_lde per job). */
 int bc_ragged_run_plain(const bc_ragged_job *jobs, uint32_t num_jobs, void *stream);
 
+/* Seeded inputs on the device: column c (c = 0..num_cols-1) at
+ * out + c * stride gets 2^log_n u64 draws of the splitmix64 stream seeded with
+ * seed * 0x9E3779B97F4A7C15 + first_col + c (raw u64, any value, read mod p).
+ * No reference counterpart: the reference's tests draw inputs on the host. */
+int bc_fill_seeded(uint64_t *out, uint32_t log_n, uint32_t num_cols, uint64_t stride, uint64_t seed,
+                   uint32_t first_col, void *stream);
+
+/* ---- Serving loop: continuous batching under an HBM budget ----
+ * No reference counterpart.  A server owns one HBM arena of hbm_budget_bytes
+ * and a scheduler thread.  Requests (NTT / iNTT / coset-LDE of num_cols
+ * columns) are split by columns into parts of at most max_part_elems
+ * transformed elements; parts are admitted first-come first-served when their
+ * buffers fit in the arena, gathered into ragged batches (one persistent
+ * launch each) and run on up to max_batches_in_flight streams at once, so
+ * requests that arrive while others run join the next batch and their work
+ * overlaps on the device.  Results do not depend on how requests were split
+ * or batched. */
+typedef struct bc_server bc_server;
+
+typedef struct {
+  uint64_t hbm_budget_bytes;      /* arena size */
+  uint64_t max_part_elems;        /* transformed elements per part (0: 2^28) */
+  uint32_t max_batches_in_flight; /* streams (0: 3) */
+  uint32_t max_parts_per_batch;   /* (0: 64) */
+} bc_server_config;
+
+typedef struct {
+  uint32_t op;             /* BC_OP_NTT, BC_OP_INTT or BC_OP_LDE (as bc_ragged_job) */
+  uint32_t log_n;          /* 8 <= log_n, log_n + log_blowup <= 32 */
+  uint32_t num_cols;
+  uint32_t log_blowup;     /* BC_OP_LDE only, 1..8 */
+  const uint64_t *inputs;  /* device columns (stride_in apart), valid until the request completes;
+                              NULL: seeded inputs as bc_fill_seeded(seed, first_col = 0) */
+  uint64_t stride_in;
+  uint64_t seed;
+  uint32_t keep_outputs;   /* 1: outputs stay in the arena until bc_server_release;
+                              0: freed as each part completes (throughput runs) */
+  uint32_t reserved;
+} bc_request;
+
+typedef struct {
+  int64_t t_submit_ns;     /* host steady clock */
+  int64_t t_first_launch_ns;
+  int64_t t_done_ns;
+  uint32_t parts;
+  uint32_t parts_done;
+  int32_t status;          /* 0 running or queued, 1 done, < 0 failed */
+  uint32_t reserved;
+} bc_request_info;
+
+typedef struct {
+  uint64_t *outputs;       /* column first_col + i at outputs + i * stride_out */
+  uint64_t stride_out;
+  uint32_t first_col;
+  uint32_t num_cols;
+} bc_part_info;
+
+typedef struct {
+  uint64_t requests_done;
+  uint64_t parts_done;
+  uint64_t batches_launched;
+  uint32_t max_batches_in_flight_seen; /* > 1: batches overlapped on the device */
+  uint32_t pending_parts;
+  uint64_t arena_high_water_bytes;
+  uint64_t arena_bytes_in_use;
+} bc_server_stats;
+
+/* Needs a live context on the current device; the server uses that device. */
+int bc_server_create(const bc_server_config *config, bc_server **out);
+/* Queues a request; *id identifies it.  Returns BC_ERROR_INVALID_VALUE for bad
+ * arguments and 2 (hipErrorOutOfMemory) when one column, or all kept outputs,
+ * cannot fit in the budget. */
+int bc_server_submit(bc_server *server, const bc_request *request, uint64_t *id);
+/* Blocks until the request has completed (or failed). */
+int bc_server_wait(bc_server *server, uint64_t id);
+int bc_server_request_info(bc_server *server, uint64_t id, bc_request_info *info);
+/* Where part `part` of a completed keep_outputs request left its outputs:
+ * the request's layout (bit-reversed LDE blocks etc.) as bc_ragged_job. */
+int bc_server_part(bc_server *server, uint64_t id, uint32_t part, bc_part_info *info);
+/* Frees a completed request's kept outputs and forgets the request. */
+int bc_server_release(bc_server *server, uint64_t id);
+/* Blocks until every submitted request has completed. */
+int bc_server_drain(bc_server *server);
+int bc_server_stats_get(bc_server *server, bc_server_stats *stats);
+/* Drains, stops the scheduler and frees the arena. */
+int bc_server_destroy(bc_server *server);
+
 #ifdef __cplusplus
 }
 #endif
