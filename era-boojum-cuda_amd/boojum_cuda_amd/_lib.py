@@ -34,6 +34,11 @@ SIGNATURES = {
     "bc_ragged_plan_destroy": (c_int, [c_vp]),
     "bc_ragged_plan_info": (c_int, [c_vp, ctypes.POINTER(c_u64), ctypes.POINTER(c_u32), ctypes.POINTER(c_u32)]),
     "bc_ragged_run_plain": (c_int, [c_vp, c_u32, c_vp]),
+    "bc_tune_set": (c_int, [c_u32, c_u32, c_vp]),
+    "bc_tune_get": (c_int, [c_u32, c_u32, c_vp]),
+    "bc_tune_clear": (c_int, []),
+    "bc_tune_load": (c_int, [ctypes.c_char_p]),
+    "bc_tune_save": (c_int, [ctypes.c_char_p]),
     "bc_fill_seeded": (c_int, [c_vp, c_u32, c_u32, c_u64, c_u64, c_u32, c_vp]),
     "bc_server_create": (c_int, [c_vp, ctypes.POINTER(c_vp)]),
     "bc_server_submit": (c_int, [c_vp, c_vp, ctypes.POINTER(c_u64)]),

@@ -40,7 +40,8 @@ bc_context *current_context() {
 
 
 // Split of log_n >= 8 into strided passes of 4..8 bits (top index bits first)
-// and one contiguous pass of 8..12 bits.
+// and one contiguous pass of 8..12 bits: the tuned split when one is set,
+// else the fewest strided passes with the largest contiguous pass.
 struct FastPlan {
   int num_strided;
   uint32_t strided[8];
@@ -48,17 +49,22 @@ struct FastPlan {
   uint32_t contig;
 };
 
-void make_fast_plan(uint32_t log_n, FastPlan &pl) {
+void make_fast_plan(uint32_t log_n, const TuneParams &tp, FastPlan &pl) {
   pl.num_strided = 0;
   if (log_n <= TILE_LOG) {
     pl.contig = log_n;
     return;
   }
   uint32_t c = TILE_LOG, s = log_n - TILE_LOG;
-  const uint32_t k = (s + 7) / 8;
+  uint32_t k = (s + 7) / 8;
   if (s < 4 * k) {  // only when k == 1 and s < 4: shrink the contiguous pass
     c -= 4 * k - s;
     s = 4 * k;
+  }
+  if (tp.contig_log && tune_params_valid(log_n, tp)) {
+    c = tp.contig_log;
+    s = log_n - c;
+    k = tp.num_strided;
   }
   uint32_t N = log_n;
   for (uint32_t i = 0; i < k; ++i) {
@@ -109,7 +115,7 @@ int describe_fast_passes(const Tables &tab, const uint64_t *inputs, uint64_t *ou
                          uint32_t log_ext, uint32_t coset_index, PassDesc *out) {
   if (log_n < 8) return -1;
   FastPlan plan;
-  make_fast_plan(log_n, plan);
+  make_fast_plan(log_n, tuned(log_n, num_cols), plan);
   const uint64_t n = 1ull << log_n;
   FastArgs a{};
   a.tab = tab;
@@ -180,6 +186,7 @@ extern "C" int bc_context_create(uint32_t powers_of_w_coarse_log_count, uint32_t
   hipError_t err = hipGetDevice(&dev);
   if (err != hipSuccess) return err;
   if (dev < 0 || dev >= MAX_DEVICES) return BC_ERROR_INVALID_VALUE;
+  tune_autoload();
 
   std::vector<uint64_t> host((size_t)NUM_TABLES * TABLE_LEN + 2 * LVL_TW_BLOCK);
   const uint64_t w = gl::OMEGA_2_32;
@@ -274,14 +281,16 @@ extern "C" int bc_batch_ntt_internal(const uint64_t *inputs, uint64_t *outputs, 
   const int np = describe_fast_passes(ctx->tab, inputs, outputs, log_n, num_ntts, stride_in, stride_out,
                                       bitrev_inputs, inverse, log_ext, coset_index, passes);
   if (np <= 0) return BC_ERROR_INVALID_VALUE;
+  const TuneParams tp = tuned(log_n, num_ntts);
   for (int k = 0; k < np; ++k) {
     PassDesc &pd = passes[k];
     hipError_t err;
     if (pd.contig) {
-      err = launch_contig(pd.a, (int)pd.log_h, inverse, bitrev_inputs, pd.mode, s);
+      err = launch_contig(pd.a, (int)pd.log_h, inverse, bitrev_inputs, pd.mode, tp.contig_grid, s);
     } else {
       uint64_t bpw = pd.pairs * pd.positions / 2048;
       bpw = bpw < 1 ? 1 : (bpw > 16 ? 16 : bpw);
+      if (tp.strided_bpw) bpw = tp.strided_bpw;
       const uint64_t groups = (pd.pairs + bpw - 1) / bpw;
       if (pd.positions > 0x7FFFFFFFull || groups > 65535) return BC_ERROR_INVALID_VALUE;
       pd.a.blocks_per_wg = (uint32_t)bpw;

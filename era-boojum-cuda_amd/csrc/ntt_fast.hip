@@ -70,9 +70,10 @@ hipError_t strided_dir(const FastArgs &a, bool inv, bool dit, int mode, dim3 gri
 
 }  // namespace
 
-hipError_t launch_contig(const FastArgs &a, int log_h, bool inverse, bool dit, int mode, hipStream_t stream) {
+hipError_t launch_contig(const FastArgs &a, int log_h, bool inverse, bool dit, int mode, uint32_t max_wgs,
+                         hipStream_t stream) {
   if (a.tiles == 0) return hipSuccess;
-  const uint32_t max_wgs = 2048;
+  if (max_wgs == 0) max_wgs = 2048;
   const dim3 grid(a.tiles < max_wgs ? a.tiles : max_wgs);
   if (a.log_n < TILE_LOG) {  // single pass over whole columns, several per tile
     switch (log_h) {

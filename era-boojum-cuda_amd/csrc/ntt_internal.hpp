@@ -102,9 +102,25 @@ int describe_fast_passes(const Tables &tab, const uint64_t *inputs, uint64_t *ou
                          uint32_t num_cols, uint64_t stride_in, uint64_t stride_out, bool bitrev_inputs, bool inverse,
                          uint32_t log_ext, uint32_t coset_index, PassDesc *out);
 
-hipError_t launch_contig(const FastArgs &a, int log_h, bool inverse, bool dit, int mode, hipStream_t stream);
+// max_wgs: grid cap of a contiguous pass (0: built-in 2048).
+hipError_t launch_contig(const FastArgs &a, int log_h, bool inverse, bool dit, int mode, uint32_t max_wgs,
+                         hipStream_t stream);
 hipError_t launch_strided(const FastArgs &a, int log_h, bool inverse, bool dit, int mode, uint32_t positions,
                           uint32_t groups, hipStream_t stream);
+
+// Tuned launch parameters for one (log_n, column-count bucket); zero fields
+// select the built-in choice (ntt_tune.hip).
+struct TuneParams {
+  uint32_t contig_log;    // contiguous pass size; 0: built-in split
+  uint32_t num_strided;   // strided passes above it (the rest of log_n split evenly)
+  uint32_t strided_bpw;   // (column, block) pairs per strided workgroup
+  uint32_t contig_grid;   // grid cap of contiguous passes
+  uint32_t unit_contig;   // ragged: tiles per work unit
+  uint32_t unit_strided;  // ragged: pairs per work unit
+};
+TuneParams tuned(uint32_t log_n, uint32_t num_cols);
+bool tune_params_valid(uint32_t log_n, const TuneParams &p);
+void tune_autoload();
 
 constexpr uint32_t TILE_LOG = 12;
 constexpr uint32_t TILE = 1u << TILE_LOG;

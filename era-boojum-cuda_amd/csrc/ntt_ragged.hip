@@ -242,6 +242,7 @@ int ragged_build(const bc_ragged_job *jobs, uint32_t num_jobs, const Tables &tab
         stages.push_back({jb.coeffs, jb.outputs + (uint64_t)brev_bits_host(c, jb.log_blowup) * n, jb.stride_coeffs,
                           jb.stride_out, false, false, jb.log_blowup, c});
     }
+    const TuneParams tp = tuned(jb.log_n, jb.num_cols);
     int32_t stage0_last = -1;
     uint32_t stage0_passes = 0;
     for (uint32_t st = 0; st < stages.size(); ++st) {
@@ -269,13 +270,13 @@ int ragged_build(const bc_ragged_job *jobs, uint32_t num_jobs, const Tables &tab
           const bool mc = jb.log_n < TILE_LOG;
           ps.s.variant = variant_id(false, pd[k].log_h, mc, dm);
           ps.s.total = pd[k].a.tiles;
-          ps.s.per_unit = 2;
+          ps.s.per_unit = tp.unit_contig ? tp.unit_contig : 2;
           ps.s.units = (uint32_t)((ps.s.total + ps.s.per_unit - 1) / ps.s.per_unit);
           ps.s.positions = 0;
         } else {
           ps.s.variant = variant_id(true, pd[k].log_h, false, dm);
           ps.s.total = pd[k].pairs;
-          ps.s.per_unit = (uint32_t)std::min<uint64_t>(pd[k].pairs, 4);
+          ps.s.per_unit = (uint32_t)std::min<uint64_t>(pd[k].pairs, tp.unit_strided ? tp.unit_strided : 4);
           ps.s.positions = (uint32_t)pd[k].positions;
           const uint64_t groups = (ps.s.total + ps.s.per_unit - 1) / ps.s.per_unit;
           ps.s.units = (uint32_t)(groups * ps.s.positions);

@@ -158,6 +158,34 @@ int bc_ragged_run_plain(const bc_ragged_job *jobs, uint32_t num_jobs, void *stre
 int bc_fill_seeded(uint64_t *out, uint32_t log_n, uint32_t num_cols, uint64_t stride, uint64_t seed,
                    uint32_t first_col, void *stream);
 
+/* ---- Tuning database ----
+ * No reference counterpart (the reference's launch plans are the static
+ * PLANS table, src/ntt.rs:74-169).  Launch parameters are looked up per
+ * (log_n, log_cols = floor(log2(num_cols))); a missing bucket takes the
+ * nearest tuned bucket of the same log_n, and zero fields (or no entry) keep
+ * the built-in choice.  Plans change speed only: every tuned plan computes
+ * the same bits.  bc_context_create loads the database named by $BC_TUNE_DB,
+ * else tuning/mi355x.txt beside libboojum_ntt.so, once per process. */
+typedef struct {
+  uint32_t contig_log;          /* contiguous pass size 9..12 (or log_n when log_n <= 12); 0: built-in */
+  uint32_t num_strided;         /* strided passes above it, each 4..8 bits */
+  uint32_t strided_bpw;         /* (column, block) pairs per strided workgroup, <= 64 */
+  uint32_t contig_grid;         /* grid cap of contiguous passes */
+  uint32_t ragged_unit_contig;  /* ragged work unit: tiles, <= 64 */
+  uint32_t ragged_unit_strided; /* ragged work unit: (column, block) pairs, <= 64 */
+} bc_tune_params;
+
+/* Sets (params != NULL) or erases the entry of one bucket; invalid parameters
+ * return BC_ERROR_INVALID_VALUE. */
+int bc_tune_set(uint32_t log_n, uint32_t log_cols, const bc_tune_params *params);
+/* The parameters a call with this shape uses (all zero: built-in). */
+int bc_tune_get(uint32_t log_n, uint32_t num_cols, bc_tune_params *params);
+int bc_tune_clear(void);
+/* Text database, one entry per line: log_n log_cols and the six fields in
+ * order; '#' starts a comment line.  Load merges into the table. */
+int bc_tune_load(const char *path);
+int bc_tune_save(const char *path);
+
 /* ---- Serving loop: continuous batching under an HBM budget ----
  * No reference counterpart.  A server owns one HBM arena of hbm_budget_bytes
  * and a scheduler thread.  Requests (NTT / iNTT / coset-LDE of num_cols

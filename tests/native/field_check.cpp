@@ -1,0 +1,83 @@
+// CPU check of the device field arithmetic (era-boojum-cuda_amd/csrc/gl_field.hpp,
+// compiled here as host code) against 128-bit integer arithmetic mod
+// p = 2^64 - 2^32 + 1: canon, add, sub, neg, mul, mul by 2^e for all 192
+// exponents (2 has order 192), pow and inv, on edge values and random draws
+// biased toward carries.  Prints "OK" and exits 0 when every result matches.
+#include "../../era-boojum-cuda_amd/csrc/gl_field.hpp"
+
+#include <stdio.h>
+
+#include <random>
+#include <utility>
+
+typedef unsigned __int128 u128;
+static const uint64_t P = gl::P;
+
+static uint64_t ref_mul(uint64_t a, uint64_t b) { return (uint64_t)(((u128)a * b) % P); }
+static uint64_t ref_pow(uint64_t a, uint64_t e) {
+  uint64_t r = 1, b = a % P;
+  for (; e; e >>= 1, b = ref_mul(b, b))
+    if (e & 1) r = ref_mul(r, b);
+  return r;
+}
+
+static const uint64_t EDGE[] = {0, 1, 2, P - 2, P - 1, P, P + 1, ~0ull, 0xFFFFFFFFull, 0x100000000ull, 1ull << 63,
+                                P - 0x100000000ull, 0xFFFFFFFF00000000ull, 0xFFFFFFFEull, 0xFFFFFFFEFFFFFFFFull};
+static const int NEDGE = sizeof(EDGE) / sizeof(EDGE[0]);
+
+static uint64_t draw(std::mt19937_64 &g, int i) {
+  if (i < NEDGE) return EDGE[i];
+  switch (i % 5) {
+  case 0: return g();
+  case 1: return g() & 0xFFFFFFFFull;
+  case 2: return g() << 32;
+  case 3: return ~(g() & 0xFFFFFFFFFFull);
+  default: return P - 1 - (g() & 0xFFFF);
+  }
+}
+
+static int bad = 0;
+static void report(const char *what, uint64_t a, uint64_t b, uint64_t got, uint64_t want) {
+  if (bad++ < 8)
+    printf("%s a=%016llx b=%016llx got %016llx want %016llx\n", what, (unsigned long long)a, (unsigned long long)b,
+           (unsigned long long)got, (unsigned long long)want);
+}
+
+template <unsigned E> static void check_pow2(std::mt19937_64 &g, int samples) {
+  const uint64_t w = ref_pow(2, E);
+  for (int i = 0; i < samples; ++i) {
+    uint64_t x = draw(g, i);
+    if (E % 96 == 0) x %= P;  // 2^0 passes x through and 2^96 = -1 negates it: both expect canonical x
+    const uint64_t got = gl::mul_pow2<E>(x), want = ref_mul(x, w);
+    if (got != want) report("mul_pow2", x, E, got, want);
+  }
+}
+
+template <unsigned... Es> static void check_all_pow2(std::mt19937_64 &g, int samples, std::integer_sequence<unsigned, Es...>) {
+  (check_pow2<Es>(g, samples), ...);
+}
+
+int main() {
+  std::mt19937_64 g(12345);
+  check_all_pow2(g, 100000, std::make_integer_sequence<unsigned, 192>{});
+  for (int i = 0; i < 2000000; ++i) {
+    const uint64_t a = draw(g, i), b = draw(g, i / 3 + 7);
+    const uint64_t ca = a % P, cb = b % P;
+    if (gl::canon(a) != ca) report("canon", a, 0, gl::canon(a), ca);
+    if (gl::add(ca, cb) != (uint64_t)(((u128)ca + cb) % P)) report("add", ca, cb, gl::add(ca, cb), 0);
+    if (gl::sub(ca, cb) != (uint64_t)(((u128)ca + P - cb) % P)) report("sub", ca, cb, gl::sub(ca, cb), 0);
+    if (gl::neg(ca) != (P - ca) % P) report("neg", ca, 0, gl::neg(ca), (P - ca) % P);
+    if (gl::mul(a, b) != ref_mul(a, b)) report("mul", a, b, gl::mul(a, b), ref_mul(a, b));
+  }
+  for (int i = 0; i < 2000; ++i) {
+    const uint64_t a = draw(g, i) % P, e = g();
+    if (gl::pow(a, e) != ref_pow(a, e)) report("pow", a, e, gl::pow(a, e), ref_pow(a, e));
+    if (a && ref_mul(gl::inv(a), a) != 1) report("inv", a, 0, gl::inv(a), 0);
+  }
+  if (bad) {
+    printf("FAIL %d mismatches\n", bad);
+    return 1;
+  }
+  printf("OK\n");
+  return 0;
+}

@@ -1,0 +1,91 @@
+"""CPU tests of the tuning database (bc_tune_*): host-only entry points, so
+the library is loaded and called without a GPU."""
+import os
+
+import pytest
+
+
+@pytest.fixture
+def tune():
+    from boojum_cuda_amd import autotune
+    autotune.clear()
+    yield autotune
+    autotune.clear()
+
+
+def test_splits_cover_valid_plans():
+    from boojum_cuda_amd import autotune
+    assert autotune.splits(12) == [(12, 0)]
+    assert autotune.splits(13) == [(9, 1)]
+    assert (12, 1) in autotune.splits(20) and (10, 2) in autotune.splits(20) and (12, 2) in autotune.splits(20)
+    for log_n in range(13, 33):
+        for c, k in autotune.splits(log_n):
+            assert 9 <= c <= 12 and 4 * k <= log_n - c <= 8 * k
+
+
+def test_set_get_nearest_bucket(tune):
+    P = tune.TuneParams
+    assert tune.get_entry(20, 256).key() == (0,) * 6
+    tune.set_entry(20, 8, P(10, 2, 4, 1024, 2, 4))
+    assert tune.get_entry(20, 256).key() == (10, 2, 4, 1024, 2, 4)
+    assert tune.get_entry(20, 300).key() == (10, 2, 4, 1024, 2, 4)   # bucket 8
+    assert tune.get_entry(20, 1).key() == (10, 2, 4, 1024, 2, 4)     # nearest bucket of log_n 20
+    assert tune.get_entry(21, 256).key() == (0,) * 6                  # other log_n: built-in
+    tune.set_entry(20, 0, P(12, 1, 0, 0, 0, 0))
+    assert tune.get_entry(20, 1).key()[:2] == (12, 1)
+    assert tune.get_entry(20, 16).key()[:2] == (12, 1)                # bucket 4: 0 and 8 tie, smaller wins
+    tune.set_entry(20, 0, None)
+    assert tune.get_entry(20, 1).key()[:2] == (10, 2)
+
+
+@pytest.mark.parametrize("log_n,params", [
+    (20, (8, 2, 0, 0, 0, 0)),     # contiguous pass below 9
+    (20, (12, 3, 0, 0, 0, 0)),    # 8 bits cannot make 3 strided passes of >= 4
+    (20, (12, 0, 0, 0, 0, 0)),    # strided bits left over
+    (13, (12, 1, 0, 0, 0, 0)),    # 1 strided bit
+    (10, (9, 0, 0, 0, 0, 0)),     # log_n <= 12 is one contiguous pass
+    (20, (0, 1, 0, 0, 0, 0)),     # strided count without a split
+    (20, (0, 0, 65, 0, 0, 0)),    # pairs per workgroup too large
+    (20, (0, 0, 0, 0, 65, 0)),    # ragged unit too large
+])
+def test_invalid_entries_rejected(tune, log_n, params):
+    with pytest.raises(RuntimeError):
+        tune.set_entry(log_n, 4, tune.TuneParams(*params))
+
+
+def test_save_load_round_trip(tune, tmp_path):
+    P = tune.TuneParams
+    entries = {(20, 8): (10, 2, 4, 1024, 2, 4), (16, 0): (10, 1, 0, 0, 1, 1), (12, 5): (12, 0, 0, 4096, 4, 0)}
+    for (log_n, lc), v in entries.items():
+        tune.set_entry(log_n, lc, P(*v))
+    path = str(tmp_path / "db.txt")
+    tune.save(path)
+    tune.clear()
+    assert tune.get_entry(20, 256).key() == (0,) * 6
+    tune.load(path)
+    for (log_n, lc), v in entries.items():
+        assert tune.get_entry(log_n, 1 << lc).key() == v
+    text = open(path).read()
+    assert text.startswith("#") and len(text.strip().splitlines()) == 1 + len(entries)
+
+
+def test_malformed_database_rejected(tune, tmp_path):
+    bad = tmp_path / "bad.txt"
+    bad.write_text("20 8 10 2 4 1024\n")          # too few fields
+    with pytest.raises(RuntimeError):
+        tune.load(str(bad))
+    bad.write_text("20 8 8 2 0 0 0 0\n")          # invalid split
+    with pytest.raises(RuntimeError):
+        tune.load(str(bad))
+    assert tune.get_entry(20, 256).key() == (0,) * 6
+    with pytest.raises(RuntimeError):
+        tune.load(str(tmp_path / "missing.txt"))
+
+
+def test_shipped_database_is_valid(tune):
+    """The committed database parses and holds only valid entries."""
+    from boojum_cuda_amd._lib import LIB_PATH
+    path = os.path.join(os.path.dirname(LIB_PATH), "tuning", "mi355x.txt")
+    if not os.path.exists(path):
+        pytest.skip("no tuning database shipped")
+    tune.load(path)
