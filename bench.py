@@ -84,8 +84,9 @@ def ragged_specs(count, log_lo, log_hi, seed):
 
 
 def bench_ragged(count, log_lo, log_hi, reps, warmup):
-    """Ragged batch: one persistent launch over every job vs the plain path
-    (separate launch sequences per job, on the same stream).  Both must give
+    """Ragged batch: the persistent work-queue kernel over every job (one
+    launch per dependency level) vs the plain path (separate launch sequences
+    per job, on the same stream).  Both must give
     the same bits.  Algorithmic bytes: 16 n (NTT/iNTT), 8 (1 + B) n (LDE)."""
     import torch
     from boojum_cuda_amd import ragged
@@ -130,7 +131,7 @@ def bench_ragged(count, log_lo, log_hi, reps, warmup):
     torch.cuda.empty_cache()
     gbs = algo / (ms_ragged * 1e-3) / 1e9
     return {"workload": f"ragged batch: {count} single-column jobs, log_n ~ U[{log_lo},{log_hi}], "
-                        f"op ~ U{{NTT, iNTT, LDE x2, LDE x4}}, one persistent launch",
+                        f"op ~ U{{NTT, iNTT, LDE x2, LDE x4}}, persistent queue kernel",
             "ms_ragged": ms_ragged, "ms_plain": ms_plain, "speedup_vs_plain": ms_plain / ms_ragged,
             "input_elems": in_elems, "output_elems": out_elems,
             "output_elems_per_s": out_elems / (ms_ragged * 1e-3),
@@ -209,6 +210,19 @@ def cpu_baseline(log_n, cols, threads):
     dt = time.perf_counter() - t0
     return {"value": cols * n / dt, "unit": "elem/s", "cores": threads, "kind": "port",
             "sample": f"{cols} columns x 2^{log_n}, forward n2b, {dt:.2f} s wall"}
+
+
+def measured_traffic(log_n, cols):
+    """HBM bytes per call from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
+    summary of this workload (copy-calibrated), or None for other shapes."""
+    path = os.path.join(ROOT, "profiles", f"ntt_{cols}x2^{log_n}_hbm_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        rec = json.load(f)
+    if rec.get("log_n") != log_n or rec.get("columns") != cols:
+        return None
+    return rec["hbm_bytes_per_call"]
 
 
 def progress(msg):
@@ -327,7 +341,7 @@ def main():
         "config": {"workload": f"batched NTT {cols} x 2^{log_n} Goldilocks, fwd/inv x n2b/b2n, out-of-place",
                    "columns": cols, "log_n": log_n, "calls_per_step": len(variants)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(log_n, cols),
                      "kernel": "one batch_ntt call (all its pass kernels)", "avg_call_ms": avg_call_ms,
                      "algorithmic_bytes_per_call": algo_bytes},
     }

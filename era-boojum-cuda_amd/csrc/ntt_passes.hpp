@@ -68,16 +68,17 @@ template <int LH, int L, int A> __device__ __forceinline__ uint32_t celem(uint32
   return (qb << LH) + (b << LM) + ((uint32_t)i << LS) + rp;
 }
 
-// Scaling factor of a contiguous-pass element at column position pos.
-template <int MODE, bool DIT>
-__device__ __forceinline__ uint64_t contig_scale(const FastArgs &a, uint32_t pos) {
-  if constexpr (MODE == MODE_LDE_IN) {
-    const uint32_t idx = DIT ? brev_bits(pos, a.log_n) : pos;  // input position holds x_idx
-    return coset_pow(a.tab, a.coset_exp, idx, false);
-  } else {
-    const uint32_t idx = DIT ? pos : brev_bits(pos, a.log_n);  // output position holds y_idx
-    return gl::mul(a.n_inv, coset_pow(a.tab, a.coset_exp, idx, true));
-  }
+// Coset scaling (MODE_LDE_IN: s^idx on the inputs; MODE_LDE_OUT: n^-1 s^-idx
+// on the outputs).  A thread's 16 points sit at indices idx0 + part(i) whose
+// parts use bits the thread's base index leaves zero, so s^(idx0 + part(i)) =
+// s^idx0 * s^part(i): the 16 part factors are computed once per workgroup
+// (into LDS, by threads 0..15) and each thread computes one power per tile.
+template <int MODE> __device__ __forceinline__ uint64_t coset_scale_pow(const FastArgs &a, uint32_t idx) {
+  return coset_pow(a.tab, a.coset_exp, idx, MODE == MODE_LDE_OUT);
+}
+template <int MODE> __device__ __forceinline__ uint64_t coset_part_factor(const FastArgs &a, uint32_t part) {
+  const uint64_t v = coset_scale_pow<MODE>(a, part);
+  return MODE == MODE_LDE_OUT ? gl::mul(a.n_inv, v) : v;
 }
 
 // Addressing of a contiguous-pass tile: 4096 consecutive points of one column,
@@ -133,6 +134,7 @@ constexpr uint32_t LDS_TW_LEN = 15u << 4;  // largest staged table (M = 256)
 struct PassLds {
   uint64_t tile[LDS_PADDED];
   uint64_t tw[LDS_TW_LEN];
+  uint64_t cf[16];  // coset part factors (LDE modes)
 };
 
 // Runs tiles tile_begin, tile_begin + step, ... < tile_end of a contiguous
@@ -158,11 +160,22 @@ __device__ __forceinline__ void contig_run(const FastArgs &a, PassLds &sh, uint3
   // level-1 twists (at most 240 entries) in LDS
   if constexpr (NL == 3) stage_level_twists<LH - 4, INV>(tw1, a.tab);
   const uint32_t rp1 = t & ((1u << (LH - 8)) - 1u);
+  // coset part factors: point i of the level-0 mapping is at tile position
+  // pos(e_0) + i 2^(LH-4); the scaled index is that position, or its log_n-bit
+  // reversal (DIT inputs, DIF outputs), whose part is brev4(i) 2^(log_n-LH)
+  constexpr bool SC = MODE == MODE_LDE_IN || MODE == MODE_LDE_OUT;
+  constexpr bool SC_BREV = (MODE == MODE_LDE_IN) == DIT;
+  if constexpr (SC) {
+    if (t < 16) {
+      const uint32_t part = SC_BREV ? (brev_bits(t, 4) << (a.log_n - LH)) : (t << (LH - 4));
+      sh.cf[t] = coset_part_factor<MODE>(a, part);
+    }
+  }
 
   uint32_t tile = tile_begin;
   uint64_t pf[16];
   if (tile < tile_end) contig_load<LH, MC>(a, tile, t, pf);
-  __syncthreads();  // tw1 staged
+  __syncthreads();  // tw1 and coset part factors staged
 
   for (; tile < tile_end; tile += step) {
     const ContigTile<LH, MC> ct(a, tile);
@@ -173,9 +186,14 @@ __device__ __forceinline__ void contig_run(const FastArgs &a, PassLds &sh, uint3
       const uint32_t next = tile + step;
       if (next < tile_end) contig_load<LH, MC>(a, next, t, pf);  // prefetch: in flight during this tile's math
     }
+    uint64_t cfb = 0;  // coset factor of the thread's point 0 in this tile
+    if constexpr (SC) {
+      const uint32_t pos = ct.pos(celem<LH, 0, 4>(t, 0, 0));
+      cfb = coset_scale_pow<MODE>(a, SC_BREV ? brev_bits(pos, a.log_n) : pos);
+    }
     if constexpr (MODE == MODE_LDE_IN) {
 #pragma unroll
-      for (int i = 0; i < 16; ++i) x[i] = gl::mul(x[i], contig_scale<MODE, DIT>(a, ct.pos(celem<LH, 0, 4>(t, 0, i))));
+      for (int i = 0; i < 16; ++i) x[i] = gl::mul(x[i], gl::mul(cfb, sh.cf[i]));
     } else {
       if (a.first) {
 #pragma unroll
@@ -218,7 +236,7 @@ __device__ __forceinline__ void contig_run(const FastArgs &a, PassLds &sh, uint3
         const uint32_t e = celem<LH, 0, 4>(t, 0, i);
         uint64_t v = lds[pidx(e)];
         if constexpr (MODE == MODE_NINV) v = gl::mul(v, a.n_inv);
-        if constexpr (MODE == MODE_LDE_OUT) v = gl::mul(v, contig_scale<MODE, DIT>(a, ct.pos(e)));
+        if constexpr (MODE == MODE_LDE_OUT) v = gl::mul(v, gl::mul(cfb, sh.cf[i]));
         bool ok;
         const uint64_t ad = ct.addr(e, a.stride_out, ct.base_out, a.num_cols, ok);
         if (ok) a.out[ad] = v;  // canonical: the field ops keep values in [0, p)
@@ -258,7 +276,7 @@ __device__ __forceinline__ void contig_run(const FastArgs &a, PassLds &sh, uint3
         const uint32_t e = celem<LH, 0, 4>(t, 0, i);
         uint64_t v = x[i];
         if constexpr (MODE == MODE_NINV) v = gl::mul(v, a.n_inv);
-        if constexpr (MODE == MODE_LDE_OUT) v = gl::mul(v, contig_scale<MODE, DIT>(a, ct.pos(e)));
+        if constexpr (MODE == MODE_LDE_OUT) v = gl::mul(v, gl::mul(cfb, sh.cf[i]));
         bool ok;
         const uint64_t ad = ct.addr(e, a.stride_out, ct.base_out, a.num_cols, ok);
         if (ok) a.out[ad] = v;
@@ -325,19 +343,13 @@ __device__ __forceinline__ void strided_run(const FastArgs &a, PassLds &sh, uint
       if constexpr (MODE == MODE_NINV) w = gl::mul(w, a.n_inv);
       pt[g * (1 << A1) + i] = w;
     }
-  // coset factors at column positions j * R + r0 + q (this is the top pass:
-  // one block per column), level-0 mapping
-  uint64_t cf[MODE == MODE_LDE_IN || MODE == MODE_LDE_OUT ? 16 : 1];
-  if constexpr (MODE == MODE_LDE_IN || MODE == MODE_LDE_OUT) {
-#pragma unroll
-    for (int i = 0; i < 16; ++i) {
-      const uint32_t pos = ((((uint32_t)i << LS0) + rp0) << log_r) + r0 + q0;
-      if constexpr (MODE == MODE_LDE_IN) {
-        cf[i] = coset_pow(a.tab, a.coset_exp, pos, false);  // n2b input: natural index
-      } else {
-        cf[i] = gl::mul(a.n_inv, coset_pow(a.tab, a.coset_exp, pos, true));  // b2n output: natural index
-      }
-    }
+  // coset factors at column positions (this is the top pass: one block per
+  // column) i 2^(log_N - 4) + base of the level-0 mapping (n2b inputs, b2n
+  // outputs): the part factors of i go to LDS, read after the barrier below
+  constexpr bool SC = MODE == MODE_LDE_IN || MODE == MODE_LDE_OUT;
+  uint64_t cf[SC ? 16 : 1];
+  if constexpr (SC) {
+    if (t < 16) sh.cf[t] = coset_part_factor<MODE>(a, t << (a.log_N - 4));
   }
 
   const uint32_t blocks_per_col_log = a.log_n - a.log_N;
@@ -365,7 +377,12 @@ __device__ __forceinline__ void strided_run(const FastArgs &a, PassLds &sh, uint
 
   uint64_t pf[16];
   if (p_begin < p_end) load(p_begin, pf);
-  __syncthreads();  // tw0 staged
+  __syncthreads();  // tw0 and coset part factors staged
+  if constexpr (SC) {
+    const uint64_t cfb = coset_scale_pow<MODE>(a, ((uint32_t)rp0 << log_r) + r0 + q0);
+#pragma unroll
+    for (int i = 0; i < 16; ++i) cf[i] = gl::mul(cfb, sh.cf[i]);
+  }
   for (uint64_t p = p_begin; p < p_end; ++p) {
     uint64_t x[16];
 #pragma unroll

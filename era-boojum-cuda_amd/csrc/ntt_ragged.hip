@@ -5,19 +5,23 @@
 // an LDE is an inverse transform followed by 2^k coset transforms that all
 // depend on it.  Each (job, stage, pass) is a segment of independent work
 // units (groups of contiguous tiles, or (r-range, pair group) units of a
-// strided pass).  Segments are ordered by dependency level, then by kernel
-// variant, then by job, so a unit's predecessors always have lower queue
-// positions and concurrently running workgroups mostly share code.
+// strided pass).  A segment's dependency level is its pass index along its
+// job's chain, and every segment depends only on segments of lower levels.
+// Segments are ordered by level, then by kernel variant, then by job, so the
+// units of one level are one contiguous range of the queue and concurrently
+// running workgroups mostly share code.
 //
-// Execution (device).  Workgroups pop unit indices from an atomic counter,
-// find their segment by binary search, wait (bounded) until the segment it
-// depends on has completed all its units, run the unit with the same pass
-// bodies as the standalone kernels, then publish their stores with a
-// device-scope release and bump the segment's completion counter.  Waiting
-// only ever targets units already taken by running workgroups, so the queue
-// always drains; the bound turns a planning error into a flag, not a hang.
+// Execution (device).  The persistent kernel is launched once per level (at
+// most 16 launches for any batch).  Its workgroups pop unit indices from the
+// level's atomic counter, find the unit's segment by binary search and run it
+// with the same pass bodies as the standalone kernels.  Units of one level
+// never wait for each other, so there are no spins and no device-scope
+// fences inside a launch: on gfx950 an agent-scope fence writes back and
+// invalidates the L2 of the XCD, which per unit costs more than the unit.
 #include "ntt_ragged.hpp"
 #include "ntt_passes.hpp"
+
+#include <string.h>
 
 #include <algorithm>
 #include <vector>
@@ -114,38 +118,44 @@ __device__ __forceinline__ void run_unit(const RaggedSeg *sg, uint32_t variant, 
 }
 
 struct QueueState {
-  unsigned long long head;
-  uint32_t error;
+  unsigned long long head[RAGGED_MAX_LEVELS];  // units popped so far, per level
+  uint32_t error;                              // stays 0 (see bc_ragged_status)
   uint32_t pad;
-  uint32_t done[1];  // one completion counter per segment (allocated to size)
 };
 
-__global__ void ragged_reset_kernel(QueueState *st, uint32_t nsegs) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i == 0) {
-    st->head = 0;
-    st->error = 0;
-  }
-  for (uint32_t k = i; k < nsegs; k += gridDim.x * blockDim.x) st->done[k] = 0;
+__global__ void ragged_reset_kernel(QueueState *st) {
+  const uint32_t i = threadIdx.x;
+  if (i < RAGGED_MAX_LEVELS) st->head[i] = 0;
+  if (i == 0) st->error = 0;
 }
 
+// One launch drains the units of one dependency level: segments
+// [seg_begin, seg_end) hold queue items [item_begin, item_end), and no unit of
+// a level depends on another unit of the same level, so units need no waits
+// and no device-scope fences; the launch boundary orders the levels.
+//
 // Loop shape: the only single-thread block that touches the queue sits at the
-// end of the body (publish completion + pop the next unit) and is followed by
-// a barrier in uniform control flow.  With a pop at the top of the body
-// instead, the compiler merges the two `threadIdx.x == 0` blocks across the
-// back edge, which leaves lane 0 alone on the path to the pop and the rest of
-// the workgroup in an inner loop that re-reads the old unit index forever.
-__global__ __launch_bounds__(NT, 2) void ntt_ragged_kernel(const RaggedSeg *__restrict__ segs, uint32_t nsegs,
-                                                        uint64_t total_items, QueueState *st, uint32_t spin_limit) {
+// end of the body and is followed by a barrier in uniform control flow.  With
+// a pop at the top of the body as well, the compiler merges two
+// `threadIdx.x == 0` blocks across the back edge, which leaves lane 0 alone on
+// the path to the pop and the rest of the workgroup in an inner loop that
+// re-reads the old unit index forever.
+__global__ __launch_bounds__(NT, 2) void ntt_ragged_kernel(const RaggedSeg *__restrict__ segs,
+                                                        const uint32_t *__restrict__ chunk_seg, uint32_t seg_begin,
+                                                        uint32_t seg_end, uint64_t item_begin, uint64_t item_end,
+                                                        unsigned long long *head) {
   __shared__ unsigned long long s_item;
-  if (threadIdx.x == 0) s_item = atomicAdd(&st->head, 1ull);
+  if (threadIdx.x == 0) s_item = item_begin + atomicAdd(head, 1ull);
   __syncthreads();
   for (;;) {
     const uint64_t item = s_item;
     __syncthreads();  // every thread has read s_item before it is overwritten
-    if (item >= total_items) break;
-    // segment: the last one whose first_item <= item
-    uint32_t lo = 0, hi = nsegs;
+    if (item >= item_end) break;
+    // segment: the last one whose first_item <= item, between the segments
+    // holding the first items of this chunk and of the next one
+    const uint64_t c = item >> RAGGED_CHUNK_LOG;
+    uint32_t lo = max(seg_begin, chunk_seg[c]);
+    uint32_t hi = min(seg_end, chunk_seg[c + 1] + 1);
     while (hi - lo > 1) {
       const uint32_t mid = (lo + hi) >> 1;
       if (segs[mid].first_item <= item) lo = mid;
@@ -153,30 +163,9 @@ __global__ __launch_bounds__(NT, 2) void ntt_ragged_kernel(const RaggedSeg *__re
     }
     lo = __builtin_amdgcn_readfirstlane(lo);
     const RaggedSeg &sg = segs[lo];
-    if (sg.dep >= 0) {
-      if (threadIdx.x == 0) {
-        uint32_t spins = 0;
-        while (__hip_atomic_load(&st->done[sg.dep], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < sg.dep_units) {
-          __builtin_amdgcn_s_sleep(8);
-          // give up at the bound, or at once when another unit already has
-          if (++spins > spin_limit ||
-              __hip_atomic_load(&st->error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
-            __hip_atomic_store(&st->error, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            break;
-          }
-        }
-      }
-      __syncthreads();
-      __threadfence();  // acquire: the predecessor's stores are visible to every wave
-    }
     run_unit(&sg, sg.variant, sg.total, sg.per_unit, sg.positions, item - sg.first_item);
-    __threadfence();  // release this thread's stores device-wide
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      __threadfence();
-      __hip_atomic_fetch_add(&st->done[lo], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-      s_item = atomicAdd(&st->head, 1ull);
-    }
+    __syncthreads();  // the unit's LDS use is over before the next unit starts
+    if (threadIdx.x == 0) s_item = item_begin + atomicAdd(head, 1ull);
     __syncthreads();
   }
 }
@@ -185,6 +174,16 @@ uint32_t brev_bits_host(uint32_t x, uint32_t bits) {
   uint32_t r = 0;
   for (uint32_t i = 0; i < bits; ++i) r |= ((x >> i) & 1u) << (bits - 1 - i);
   return r;
+}
+
+// Work-unit size when none is tuned: `large` items (tiles or pairs) per unit,
+// halved while the segment would have fewer than 64 units (items / size units
+// at each of `positions` r-ranges), so that small segments still spread over
+// many workgroups.
+uint32_t default_unit(uint64_t items, uint64_t positions, uint32_t large) {
+  uint32_t u = large;
+  while (u > 1 && ((items + u - 1) / u) * positions < 64) u /= 2;
+  return u;
 }
 
 struct PlanSeg {
@@ -270,13 +269,14 @@ int ragged_build(const bc_ragged_job *jobs, uint32_t num_jobs, const Tables &tab
           const bool mc = jb.log_n < TILE_LOG;
           ps.s.variant = variant_id(false, pd[k].log_h, mc, dm);
           ps.s.total = pd[k].a.tiles;
-          ps.s.per_unit = tp.unit_contig ? tp.unit_contig : 2;
+          ps.s.per_unit = tp.unit_contig ? tp.unit_contig : default_unit(ps.s.total, 1, 4);
           ps.s.units = (uint32_t)((ps.s.total + ps.s.per_unit - 1) / ps.s.per_unit);
           ps.s.positions = 0;
         } else {
           ps.s.variant = variant_id(true, pd[k].log_h, false, dm);
           ps.s.total = pd[k].pairs;
-          ps.s.per_unit = (uint32_t)std::min<uint64_t>(pd[k].pairs, tp.unit_strided ? tp.unit_strided : 4);
+          ps.s.per_unit = (uint32_t)std::min<uint64_t>(
+              pd[k].pairs, tp.unit_strided ? tp.unit_strided : default_unit(pd[k].pairs, pd[k].positions, 8));
           ps.s.positions = (uint32_t)pd[k].positions;
           const uint64_t groups = (ps.s.total + ps.s.per_unit - 1) / ps.s.per_unit;
           ps.s.units = (uint32_t)(groups * ps.s.positions);
@@ -304,6 +304,8 @@ int ragged_build(const bc_ragged_job *jobs, uint32_t num_jobs, const Tables &tab
   std::vector<uint32_t> pos_of(segs.size());
   for (uint32_t i = 0; i < order.size(); ++i) pos_of[order[i]] = i;
   out.segs.resize(segs.size());
+  out.level_seg.clear();
+  out.level_item.clear();
   uint64_t item = 0;
   for (uint32_t i = 0; i < order.size(); ++i) {
     const PlanSeg &ps = segs[order[i]];
@@ -313,21 +315,48 @@ int ragged_build(const bc_ragged_job *jobs, uint32_t num_jobs, const Tables &tab
     if (ps.dep_key >= 0) {
       r.dep = (int32_t)pos_of[ps.dep_key];
       r.dep_units = segs[ps.dep_key].s.units;
-      if ((uint32_t)r.dep >= i) return BC_ERROR_INVALID_VALUE;  // ordering invariant
+      if (segs[ps.dep_key].level >= ps.level) return BC_ERROR_INVALID_VALUE;  // levels order the launches
     } else {
       r.dep = -1;
       r.dep_units = 0;
     }
+    while (out.level_seg.size() <= ps.level) {  // first segment of each level
+      out.level_seg.push_back(i);
+      out.level_item.push_back(r.first_item);
+    }
     out.levels = std::max(out.levels, ps.level + 1);
     out.segs[i] = r;
   }
+  if (out.levels > RAGGED_MAX_LEVELS) return BC_ERROR_INVALID_VALUE;
+  out.level_seg.push_back((uint32_t)out.segs.size());
+  out.level_item.push_back(item);
   out.total_items = item;
+  // chunk index: segment of item c << RAGGED_CHUNK_LOG for every chunk, and one
+  // more entry so that chunk_seg[c + 1] exists for the last chunk
+  const uint64_t chunks = (item >> RAGGED_CHUNK_LOG) + 1;
+  out.chunk_seg.assign(chunks + 1, out.segs.empty() ? 0u : (uint32_t)out.segs.size() - 1);
+  uint32_t sgi = 0;
+  for (uint64_t c = 0; c < chunks && !out.segs.empty(); ++c) {
+    const uint64_t it = c << RAGGED_CHUNK_LOG;
+    if (it >= item) break;
+    while (sgi + 1 < out.segs.size() && out.segs[sgi + 1].first_item <= it) ++sgi;
+    out.chunk_seg[c] = sgi;
+  }
   return BC_SUCCESS;
 }
 
-size_t ragged_state_bytes(uint32_t nsegs) {
-  return sizeof(QueueState) + std::max<size_t>(1, nsegs) * sizeof(uint32_t);
+size_t ragged_table_bytes(const RaggedHostPlan &hp) {
+  return std::max<size_t>(1, hp.segs.size()) * sizeof(RaggedSeg) + hp.chunk_seg.size() * sizeof(uint32_t);
 }
+
+void ragged_pack_table(const RaggedHostPlan &hp, void *host_buf) {
+  uint8_t *b = (uint8_t *)host_buf;
+  if (!hp.segs.empty()) memcpy(b, hp.segs.data(), hp.segs.size() * sizeof(RaggedSeg));
+  memcpy(b + std::max<size_t>(1, hp.segs.size()) * sizeof(RaggedSeg), hp.chunk_seg.data(),
+         hp.chunk_seg.size() * sizeof(uint32_t));
+}
+
+size_t ragged_state_bytes() { return sizeof(QueueState); }
 
 uint32_t ragged_workgroups(int dev) {
   int cus = 0, per_cu = 0;
@@ -337,17 +366,23 @@ uint32_t ragged_workgroups(int dev) {
   return (uint32_t)(cus * per_cu);
 }
 
-hipError_t ragged_launch(const RaggedSeg *d_segs, uint32_t nsegs, uint64_t total_items, void *d_state,
-                         uint32_t workgroups, hipStream_t s) {
+hipError_t ragged_launch(const void *d_table, const RaggedHostPlan &hp, void *d_state, uint32_t workgroups,
+                         hipStream_t s) {
   QueueState *st = (QueueState *)d_state;
-  ragged_reset_kernel<<<dim3((nsegs + 255) / 256 + 1), dim3(256), 0, s>>>(st, nsegs);
+  const RaggedSeg *d_segs = (const RaggedSeg *)d_table;
+  const uint32_t *d_chunks = (const uint32_t *)((const uint8_t *)d_table +
+                                                std::max<size_t>(1, hp.segs.size()) * sizeof(RaggedSeg));
+  ragged_reset_kernel<<<dim3(1), dim3(64), 0, s>>>(st);
   hipError_t err = hipGetLastError();
-  if (err != hipSuccess) return err;
-  if (total_items == 0) return hipSuccess;
-  const uint32_t wgs = (uint32_t)std::min<uint64_t>(std::max(1u, workgroups), total_items);
-  // ~0.2 us per sleep: the bound is seconds, far beyond any real wait
-  ntt_ragged_kernel<<<dim3(wgs), dim3(NT), 0, s>>>(d_segs, nsegs, total_items, st, 1u << 24);
-  return hipGetLastError();
+  for (uint32_t l = 0; l < hp.levels && err == hipSuccess; ++l) {
+    const uint64_t units = hp.level_item[l + 1] - hp.level_item[l];
+    if (units == 0) continue;
+    const uint32_t wgs = (uint32_t)std::min<uint64_t>(std::max(1u, workgroups), units);
+    ntt_ragged_kernel<<<dim3(wgs), dim3(NT), 0, s>>>(d_segs, d_chunks, hp.level_seg[l], hp.level_seg[l + 1],
+                                                     hp.level_item[l], hp.level_item[l + 1], &st->head[l]);
+    err = hipGetLastError();
+  }
+  return err;
 }
 
 uint32_t *ragged_error_ptr(void *d_state) { return &((QueueState *)d_state)->error; }
@@ -360,11 +395,9 @@ hipError_t ragged_read_error(const void *d_state, uint32_t *error) {
 
 struct bc_ragged_plan {
   int device;
-  uint32_t nsegs;
-  uint64_t total_items;
-  uint32_t levels;
   uint32_t workgroups;
-  bc::RaggedSeg *d_segs;
+  bc::RaggedHostPlan hp;  // level ranges (the packed table itself is on the device)
+  void *d_table;
   void *d_state;
 };
 
@@ -388,17 +421,16 @@ extern "C" int bc_ragged_plan_create(const bc_ragged_job *jobs, uint32_t num_job
   if (err != hipSuccess) return err;
   bc_ragged_plan *p = new bc_ragged_plan{};
   p->device = dev;
-  p->nsegs = (uint32_t)hp.segs.size();
-  p->total_items = hp.total_items;
-  p->levels = hp.levels;
   p->workgroups = ragged_workgroups(dev);
-  const size_t seg_bytes = std::max<size_t>(1, hp.segs.size()) * sizeof(RaggedSeg);
-  err = hipMalloc((void **)&p->d_segs, seg_bytes);
-  if (err == hipSuccess) err = hipMalloc(&p->d_state, ragged_state_bytes(p->nsegs));
-  if (err == hipSuccess && !hp.segs.empty())
-    err = hipMemcpy(p->d_segs, hp.segs.data(), hp.segs.size() * sizeof(RaggedSeg), hipMemcpyHostToDevice);
+  const size_t table_bytes = ragged_table_bytes(hp);
+  std::vector<uint64_t> packed((table_bytes + 7) / 8);
+  ragged_pack_table(hp, packed.data());
+  err = hipMalloc(&p->d_table, table_bytes);
+  if (err == hipSuccess) err = hipMalloc(&p->d_state, ragged_state_bytes());
+  if (err == hipSuccess) err = hipMemcpy(p->d_table, packed.data(), table_bytes, hipMemcpyHostToDevice);
+  p->hp = std::move(hp);
   if (err != hipSuccess) {
-    (void)hipFree(p->d_segs);
+    (void)hipFree(p->d_table);
     (void)hipFree(p->d_state);
     delete p;
     return err;
@@ -409,7 +441,7 @@ extern "C" int bc_ragged_plan_create(const bc_ragged_job *jobs, uint32_t num_job
 
 extern "C" int bc_ragged_run(bc_ragged_plan *p, void *stream) {
   if (!p) return BC_ERROR_INVALID_VALUE;
-  return bc::ragged_launch(p->d_segs, p->nsegs, p->total_items, p->d_state, p->workgroups, (hipStream_t)stream);
+  return bc::ragged_launch(p->d_table, p->hp, p->d_state, p->workgroups, (hipStream_t)stream);
 }
 
 extern "C" int bc_ragged_status(bc_ragged_plan *p, int *failed) {
@@ -423,15 +455,15 @@ extern "C" int bc_ragged_status(bc_ragged_plan *p, int *failed) {
 
 extern "C" int bc_ragged_plan_info(const bc_ragged_plan *p, uint64_t *units, uint32_t *levels, uint32_t *workgroups) {
   if (!p) return BC_ERROR_INVALID_VALUE;
-  if (units) *units = p->total_items;
-  if (levels) *levels = p->levels;
+  if (units) *units = p->hp.total_items;
+  if (levels) *levels = p->hp.levels;
   if (workgroups) *workgroups = p->workgroups;
   return BC_SUCCESS;
 }
 
 extern "C" int bc_ragged_plan_destroy(bc_ragged_plan *p) {
   if (!p) return BC_ERROR_INVALID_VALUE;
-  hipError_t e1 = hipFree(p->d_segs);
+  hipError_t e1 = hipFree(p->d_table);
   hipError_t e2 = hipFree(p->d_state);
   delete p;
   return e1 != hipSuccess ? e1 : e2;

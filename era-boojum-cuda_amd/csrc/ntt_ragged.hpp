@@ -15,14 +15,24 @@ struct RaggedSeg {
   uint64_t total;       // contiguous: tiles; strided: (column, block) pairs
   uint32_t units;
   uint32_t variant;
-  int32_t dep;          // segment that must be complete first, -1 if none
-  uint32_t dep_units;
+  int32_t dep;          // segment that must be complete first, -1 if none (planning check;
+  uint32_t dep_units;   // the level launches enforce the order on the device)
   uint32_t per_unit;    // contiguous: tiles per unit; strided: pairs per unit
   uint32_t positions;   // strided: r-ranges
 };
 
+// Dependency levels of a plan: pass k of a job's first stage is level k; the
+// coset stages of an LDE follow the inverse stage's last level.
+constexpr uint32_t RAGGED_MAX_LEVELS = 16;
+// Queue items per entry of the chunk index (segment lookup without a full
+// binary search).
+constexpr uint32_t RAGGED_CHUNK_LOG = 6;
+
 struct RaggedHostPlan {
-  std::vector<RaggedSeg> segs;  // in queue order
+  std::vector<RaggedSeg> segs;        // in queue order: by level, then kernel variant, then job
+  std::vector<uint32_t> level_seg;    // levels + 1 entries: first segment of each level, then segs.size()
+  std::vector<uint64_t> level_item;   // levels + 1 entries: first queue item of each level, then total_items
+  std::vector<uint32_t> chunk_seg;    // entry c: segment holding item c << RAGGED_CHUNK_LOG; last entry: segs.size() - 1
   uint64_t total_items = 0;
   uint32_t levels = 0;
 };
@@ -30,14 +40,19 @@ struct RaggedHostPlan {
 int ragged_validate_job(const bc_ragged_job &j);
 // Host planning only (no HIP calls).
 int ragged_build(const bc_ragged_job *jobs, uint32_t num_jobs, const Tables &tab, RaggedHostPlan &out);
-// Device bytes of the queue state (head, error flag, one counter per segment).
-size_t ragged_state_bytes(uint32_t nsegs);
+// Device bytes of the queue state (one head per level, error flag).
+size_t ragged_state_bytes();
 // Persistent workgroups that fit on device `dev` at once.
 uint32_t ragged_workgroups(int dev);
-// Resets the queue state and launches the persistent kernel on `s`; the
-// segments must already be in device memory.  Allocates nothing.
-hipError_t ragged_launch(const RaggedSeg *d_segs, uint32_t nsegs, uint64_t total_items, void *d_state,
-                         uint32_t workgroups, hipStream_t s);
+// Device image of a plan: the segment table followed by the chunk index, in
+// one buffer of ragged_table_bytes(hp) bytes (8-byte aligned).
+size_t ragged_table_bytes(const RaggedHostPlan &hp);
+void ragged_pack_table(const RaggedHostPlan &hp, void *host_buf);
+// Resets the queue state and launches the persistent kernel once per
+// dependency level on `s`; the packed table must already be in device memory
+// at d_table.  Allocates nothing.
+hipError_t ragged_launch(const void *d_table, const RaggedHostPlan &hp, void *d_state, uint32_t workgroups,
+                         hipStream_t s);
 // Device address of the error flag inside the queue state.
 uint32_t *ragged_error_ptr(void *d_state);
 // Synchronous read of the error flag of a completed run.

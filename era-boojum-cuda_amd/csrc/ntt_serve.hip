@@ -73,6 +73,8 @@ int64_t now_ns() {
 
 constexpr uint64_t ARENA_ALIGN = 64 * 1024;
 constexpr uint32_t SLOT_MAX_SEGS = 8192;
+// segment table plus a chunk index of up to 2^18 entries (2^24 queue items)
+constexpr size_t SLOT_TABLE_BYTES = SLOT_MAX_SEGS * sizeof(RaggedSeg) + (1u << 18) * sizeof(uint32_t);
 constexpr int32_t STATUS_QUEUE_ERROR = 702;  // hipErrorLaunchTimeOut: a unit gave up waiting
 
 struct Request;
@@ -100,9 +102,9 @@ struct Request {
 struct Slot {
   hipStream_t stream = nullptr;
   hipEvent_t done = nullptr;
-  RaggedSeg *d_segs = nullptr;
+  void *d_table = nullptr;      // packed segment table and chunk index
   void *d_state = nullptr;
-  RaggedSeg *h_segs = nullptr;  // pinned staging of the segment table
+  void *h_table = nullptr;      // pinned staging of the table
   uint32_t *h_error = nullptr;  // pinned copy of the queue's error flag
   bool busy = false;
   std::vector<Part *> parts;
@@ -214,16 +216,14 @@ bool bc_server::launch_batch(bc::Slot &sl) {
   }
   RaggedHostPlan hp;
   int rc = ragged_build(jobs.data(), (uint32_t)jobs.size(), tab, hp);
-  if (rc == BC_SUCCESS && hp.segs.size() > SLOT_MAX_SEGS) rc = BC_ERROR_INVALID_VALUE;
+  if (rc == BC_SUCCESS && ragged_table_bytes(hp) > SLOT_TABLE_BYTES) rc = BC_ERROR_INVALID_VALUE;
   if (rc != BC_SUCCESS) {
     finish_parts(parts, rc);
     return true;
   }
-  hipError_t err = hipSuccess;
-  if (!hp.segs.empty()) {
-    memcpy(sl.h_segs, hp.segs.data(), hp.segs.size() * sizeof(RaggedSeg));
-    err = hipMemcpyAsync(sl.d_segs, sl.h_segs, hp.segs.size() * sizeof(RaggedSeg), hipMemcpyHostToDevice, sl.stream);
-  }
+  const size_t table_bytes = ragged_table_bytes(hp);
+  ragged_pack_table(hp, sl.h_table);
+  hipError_t err = hipMemcpyAsync(sl.d_table, sl.h_table, table_bytes, hipMemcpyHostToDevice, sl.stream);
   for (size_t k = 0; k < parts.size() && err == hipSuccess; ++k) {
     const Part &p = *parts[k];
     if (!p.req->r.inputs)
@@ -231,7 +231,7 @@ bool bc_server::launch_batch(bc::Slot &sl) {
                         p.req->r.seed, p.first_col, sl.stream);
   }
   if (err == hipSuccess)
-    err = ragged_launch(sl.d_segs, (uint32_t)hp.segs.size(), hp.total_items, sl.d_state, workgroups, sl.stream);
+    err = ragged_launch(sl.d_table, hp, sl.d_state, workgroups, sl.stream);
   if (err == hipSuccess)
     err = hipMemcpyAsync(sl.h_error, ragged_error_ptr(sl.d_state), sizeof(uint32_t), hipMemcpyDeviceToHost,
                          sl.stream);
@@ -297,9 +297,9 @@ void destroy_server(bc_server *s) {
     if (sl.stream) (void)hipStreamSynchronize(sl.stream);
     if (sl.done) (void)hipEventDestroy(sl.done);
     if (sl.stream) (void)hipStreamDestroy(sl.stream);
-    if (sl.d_segs) (void)hipFree(sl.d_segs);
+    if (sl.d_table) (void)hipFree(sl.d_table);
     if (sl.d_state) (void)hipFree(sl.d_state);
-    if (sl.h_segs) (void)hipHostFree(sl.h_segs);
+    if (sl.h_table) (void)hipHostFree(sl.h_table);
     if (sl.h_error) (void)hipHostFree(sl.h_error);
   }
   if (s->arena_base) (void)hipFree(s->arena_base);
@@ -336,9 +336,9 @@ extern "C" int bc_server_create(const bc_server_config *config, bc_server **out)
   for (Slot &sl : s->slots) {
     if (err == hipSuccess) err = hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking);
     if (err == hipSuccess) err = hipEventCreateWithFlags(&sl.done, hipEventDisableTiming);
-    if (err == hipSuccess) err = hipMalloc((void **)&sl.d_segs, SLOT_MAX_SEGS * sizeof(RaggedSeg));
-    if (err == hipSuccess) err = hipMalloc(&sl.d_state, ragged_state_bytes(SLOT_MAX_SEGS));
-    if (err == hipSuccess) err = hipHostMalloc((void **)&sl.h_segs, SLOT_MAX_SEGS * sizeof(RaggedSeg), 0);
+    if (err == hipSuccess) err = hipMalloc(&sl.d_table, SLOT_TABLE_BYTES);
+    if (err == hipSuccess) err = hipMalloc(&sl.d_state, ragged_state_bytes());
+    if (err == hipSuccess) err = hipHostMalloc(&sl.h_table, SLOT_TABLE_BYTES, 0);
     if (err == hipSuccess) err = hipHostMalloc((void **)&sl.h_error, sizeof(uint32_t), 0);
   }
   if (err != hipSuccess) {

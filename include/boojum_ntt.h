@@ -106,7 +106,8 @@ int bc_graph_launch(bc_graph *graph, void *stream);
 int bc_graph_destroy(bc_graph *graph);
 
 /* Ragged batches: many independent jobs of different sizes and operations,
- * run by one persistent kernel that pulls work units from a device queue.
+ * run by one persistent kernel that pulls work units from a device queue
+ * (launched once per dependency level: pass k of every job together).
  * Each job transforms num_cols columns of 2^log_n points (8 <= log_n):
  *   BC_OP_NTT   forward, natural -> bit-reversed (bc_batch_ntt_internal with
  *               bitrev_inputs = false, inverse = false)
@@ -138,11 +139,12 @@ typedef struct bc_ragged_plan bc_ragged_plan;
 /* Plans the jobs into work units (host + one device upload).  The plan keeps
  * the job pointers; buffers must stay valid while it is run. */
 int bc_ragged_plan_create(const bc_ragged_job *jobs, uint32_t num_jobs, bc_ragged_plan **out);
-/* Resets the queue and runs every job with one persistent launch on stream. */
+/* Resets the queue and runs every job on stream (one persistent launch per
+ * dependency level, at most 16). */
 int bc_ragged_run(bc_ragged_plan *plan, void *stream);
-/* After the stream has completed a run: *failed = 1 if a work unit gave up
- * waiting for its predecessor (a bounded wait that never expires in a
- * correct plan); the outputs are then not valid. */
+/* After the stream has completed a run: *failed = 1 if the device queue
+ * reported an error.  No work unit waits on another inside a launch, so the
+ * current kernel never sets it; the call stays for callers that check it. */
 int bc_ragged_status(bc_ragged_plan *plan, int *failed);
 int bc_ragged_plan_destroy(bc_ragged_plan *plan);
 /* Statistics of a plan: work units, dependency levels, persistent workgroups. */
