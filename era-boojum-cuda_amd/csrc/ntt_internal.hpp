@@ -58,6 +58,13 @@ struct PassArgs {
   uint32_t coset_exp;  // coset_index << (32 - log_n - log_ext)
 };
 
+// Per-column base pointers of one pass (ragged job groups: the columns of
+// several jobs, each with its own buffers, processed as one batch).
+struct ColPtrs {
+  const uint64_t *in;
+  uint64_t *out;
+};
+
 // Arguments of the register-resident pass kernels (ntt_fast.hip).
 struct FastArgs {
   const uint64_t *in;
@@ -75,6 +82,7 @@ struct FastArgs {
   uint32_t last;           // pass writes the final (canonical) output
   uint32_t tiles;          // contiguous passes: number of 4096-point tiles
   uint32_t blocks_per_wg;  // strided passes: (column, block) pairs per workgroup
+  const ColPtrs *cols;     // column-table addressing (ragged): column c at cols[c], in/out/strides unused
 };
 
 // Scaling folded into a pass (see plan in ntt_api.hip).

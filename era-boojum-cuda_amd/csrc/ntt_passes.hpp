@@ -81,34 +81,48 @@ template <int MODE> __device__ __forceinline__ uint64_t coset_part_factor(const 
   return MODE == MODE_LDE_OUT ? gl::mul(a.n_inv, v) : v;
 }
 
+// Column addressing: one matrix (base + c * stride) for the standalone
+// launches, or a per-column pointer table (CT) for ragged job groups.
+template <bool CT> __device__ __forceinline__ const uint64_t *in_col(const FastArgs &a, uint64_t c) {
+  if constexpr (CT) return a.cols[c].in;
+  else return a.in + c * a.stride_in;
+}
+template <bool CT> __device__ __forceinline__ uint64_t *out_col(const FastArgs &a, uint64_t c) {
+  if constexpr (CT) return a.cols[c].out;
+  else return a.out + c * a.stride_out;
+}
+
 // Addressing of a contiguous-pass tile: 4096 consecutive points of one column,
 // or (MC: single-pass transforms shorter than 4096) 2^(12-h) whole columns.
-template <int LH, bool MC> struct ContigTile {
-  uint64_t base_in, base_out;
+template <int LH, bool MC, bool CT> struct ContigTile {
+  const uint64_t *in;  // !MC: the tile's first input point
+  uint64_t *out;       // !MC: the tile's first output point
   uint32_t pos0, col0;
-  static constexpr bool multicol = MC;
   __device__ __forceinline__ ContigTile(const FastArgs &a, uint32_t tile) {
     if constexpr (!MC) {
       const uint32_t sh = a.log_n - TILE_LOG;
       const uint64_t col = tile >> sh;
       pos0 = (tile & ((1u << sh) - 1u)) << TILE_LOG;
-      base_in = col * a.stride_in + pos0;
-      base_out = col * a.stride_out + pos0;
+      in = in_col<CT>(a, col) + pos0;
+      out = out_col<CT>(a, col) + pos0;
       col0 = 0;
     } else {
       col0 = tile << (12 - LH);
       pos0 = 0;
-      base_in = base_out = 0;
+      in = nullptr;
+      out = nullptr;
     }
   }
-  __device__ __forceinline__ uint64_t addr(uint32_t e, uint64_t stride, uint64_t base, uint32_t ncols, bool &ok) const {
-    if constexpr (!MC) {
-      ok = true;
-      return base + e;
-    }
+  // element e of the tile; nullptr for MC slots past the last column
+  __device__ __forceinline__ const uint64_t *src(const FastArgs &a, uint32_t e) const {
+    if constexpr (!MC) return in + e;
     const uint32_t c = col0 + (e >> LH);
-    ok = c < ncols;
-    return (uint64_t)c * stride + (e & ((1u << LH) - 1u));
+    return c < a.num_cols ? in_col<CT>(a, c) + (e & ((1u << LH) - 1u)) : nullptr;
+  }
+  __device__ __forceinline__ uint64_t *dst(const FastArgs &a, uint32_t e) const {
+    if constexpr (!MC) return out + e;
+    const uint32_t c = col0 + (e >> LH);
+    return c < a.num_cols ? out_col<CT>(a, c) + (e & ((1u << LH) - 1u)) : nullptr;
   }
   __device__ __forceinline__ uint32_t pos(uint32_t e) const {
     if constexpr (MC) return e & ((1u << LH) - 1u);
@@ -118,14 +132,13 @@ template <int LH, bool MC> struct ContigTile {
 
 // Loads a tile's 16 values per thread in the level-0 mapping (lanes on
 // consecutive points).
-template <int LH, bool MC>
+template <int LH, bool MC, bool CT>
 __device__ __forceinline__ void contig_load(const FastArgs &a, uint32_t tile, uint32_t t, uint64_t *x) {
-  const ContigTile<LH, MC> ct(a, tile);
+  const ContigTile<LH, MC, CT> ct(a, tile);
 #pragma unroll
   for (int i = 0; i < 16; ++i) {
-    bool ok;
-    const uint64_t ad = ct.addr(celem<LH, 0, 4>(t, 0, i), a.stride_in, ct.base_in, a.num_cols, ok);
-    x[i] = ok ? a.in[ad] : 0ull;
+    const uint64_t *q = ct.src(a, celem<LH, 0, 4>(t, 0, i));
+    x[i] = q ? *q : 0ull;
   }
 }
 
@@ -139,7 +152,7 @@ struct PassLds {
 
 // Runs tiles tile_begin, tile_begin + step, ... < tile_end of a contiguous
 // pass.  Called by a whole workgroup; ends with a barrier.
-template <int LH, bool INV, bool DIT, int MODE, bool MC>
+template <int LH, bool INV, bool DIT, int MODE, bool MC, bool CT = false>
 __device__ __forceinline__ void contig_run(const FastArgs &a, PassLds &sh, uint32_t tile_begin, uint32_t tile_end,
                                            uint32_t step) {
   constexpr int NL = LH > 8 ? 3 : 2;
@@ -174,17 +187,17 @@ __device__ __forceinline__ void contig_run(const FastArgs &a, PassLds &sh, uint3
 
   uint32_t tile = tile_begin;
   uint64_t pf[16];
-  if (tile < tile_end) contig_load<LH, MC>(a, tile, t, pf);
+  if (tile < tile_end) contig_load<LH, MC, CT>(a, tile, t, pf);
   __syncthreads();  // tw1 and coset part factors staged
 
   for (; tile < tile_end; tile += step) {
-    const ContigTile<LH, MC> ct(a, tile);
+    const ContigTile<LH, MC, CT> ct(a, tile);
     uint64_t x[16];
 #pragma unroll
     for (int i = 0; i < 16; ++i) x[i] = pf[i];
     {
       const uint32_t next = tile + step;
-      if (next < tile_end) contig_load<LH, MC>(a, next, t, pf);  // prefetch: in flight during this tile's math
+      if (next < tile_end) contig_load<LH, MC, CT>(a, next, t, pf);  // prefetch: in flight during this tile's math
     }
     uint64_t cfb = 0;  // coset factor of the thread's point 0 in this tile
     if constexpr (SC) {
@@ -237,9 +250,8 @@ __device__ __forceinline__ void contig_run(const FastArgs &a, PassLds &sh, uint3
         uint64_t v = lds[pidx(e)];
         if constexpr (MODE == MODE_NINV) v = gl::mul(v, a.n_inv);
         if constexpr (MODE == MODE_LDE_OUT) v = gl::mul(v, gl::mul(cfb, sh.cf[i]));
-        bool ok;
-        const uint64_t ad = ct.addr(e, a.stride_out, ct.base_out, a.num_cols, ok);
-        if (ok) a.out[ad] = v;  // canonical: the field ops keep values in [0, p)
+        uint64_t *q = ct.dst(a, e);
+        if (q) *q = v;  // canonical: the field ops keep values in [0, p)
       }
     } else {
       // regroup the coalesced load into the last level's runs through LDS
@@ -277,9 +289,8 @@ __device__ __forceinline__ void contig_run(const FastArgs &a, PassLds &sh, uint3
         uint64_t v = x[i];
         if constexpr (MODE == MODE_NINV) v = gl::mul(v, a.n_inv);
         if constexpr (MODE == MODE_LDE_OUT) v = gl::mul(v, gl::mul(cfb, sh.cf[i]));
-        bool ok;
-        const uint64_t ad = ct.addr(e, a.stride_out, ct.base_out, a.num_cols, ok);
-        if (ok) a.out[ad] = v;
+        uint64_t *q = ct.dst(a, e);
+        if (q) *q = v;
       }
     }
     __syncthreads();  // LDS is reused by the next tile
@@ -291,7 +302,7 @@ __device__ __forceinline__ void contig_run(const FastArgs &a, PassLds &sh, uint3
 
 // Runs (column, block) pairs [p_begin, p_end) at r-range rho of a strided
 // pass.  Called by a whole workgroup; ends with a barrier.
-template <int LH, bool INV, bool DIT, int MODE>
+template <int LH, bool INV, bool DIT, int MODE, bool CT = false>
 __device__ __forceinline__ void strided_run(const FastArgs &a, PassLds &sh, uint32_t rho, uint64_t p_begin,
                                             uint64_t p_end) {
   constexpr int LT = 12 - LH;         // log2 sub-DFTs (consecutive r) per tile
@@ -353,14 +364,13 @@ __device__ __forceinline__ void strided_run(const FastArgs &a, PassLds &sh, uint
   }
 
   const uint32_t blocks_per_col_log = a.log_n - a.log_N;
-  auto pair_off = [&](uint64_t p, uint64_t stride) -> uint64_t {
-    const uint64_t col = p >> blocks_per_col_log;
-    const uint64_t blk = p & ((1ull << blocks_per_col_log) - 1ull);
-    return col * stride + (blk << a.log_N) + r0;
+  // offset of pair p's r-range inside its column
+  auto pair_pos = [&](uint64_t p) -> uint64_t {
+    return ((p & ((1ull << blocks_per_col_log) - 1ull)) << a.log_N) + r0;
   };
   // loads one pair's 16 values in the first level's mapping
   auto load = [&](uint64_t p, uint64_t *v) {
-    const uint64_t *in = a.in + pair_off(p, a.stride_in);
+    const uint64_t *in = in_col<CT>(a, p >> blocks_per_col_log) + pair_pos(p);
     if constexpr (!DIT) {
       const uint64_t *q = in + base0;
 #pragma unroll
@@ -388,7 +398,7 @@ __device__ __forceinline__ void strided_run(const FastArgs &a, PassLds &sh, uint
 #pragma unroll
     for (int i = 0; i < 16; ++i) x[i] = pf[i];
     if (p + 1 < p_end) load(p + 1, pf);  // prefetch: in flight during this pair's math
-    uint64_t *out = a.out + pair_off(p, a.stride_out);
+    uint64_t *out = out_col<CT>(a, p >> blocks_per_col_log) + pair_pos(p);
     if constexpr (!DIT) {
       if constexpr (MODE == MODE_LDE_IN) {
 #pragma unroll

@@ -24,6 +24,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <map>
 #include <vector>
 
 namespace bc {
@@ -48,14 +49,14 @@ template <int LH, bool MC, uint32_t DM>
 __device__ __noinline__ void contig_unit(const RaggedSeg *sg, uint32_t b, uint32_t e) {
   constexpr bool INV = DM >= 2;
   constexpr int MODE = DM == 1 ? MODE_LDE_IN : (DM == 3 ? MODE_NINV : MODE_NONE);
-  contig_run<LH, INV, INV, MODE, MC>(sg->a, ragged_lds, b, e, 1);
+  contig_run<LH, INV, INV, MODE, MC, true>(sg->a, ragged_lds, b, e, 1);
 }
 
 template <int LH, uint32_t DM>
 __device__ __noinline__ void strided_unit(const RaggedSeg *sg, uint32_t rho, uint64_t b, uint64_t e) {
   constexpr bool INV = DM >= 2;
   constexpr int MODE = DM == 1 ? MODE_LDE_IN : (DM == 3 ? MODE_NINV : MODE_NONE);
-  strided_run<LH, INV, INV, MODE>(sg->a, ragged_lds, rho, b, e);
+  strided_run<LH, INV, INV, MODE, true>(sg->a, ragged_lds, rho, b, e);
 }
 
 template <int LH, bool MC>
@@ -186,10 +187,30 @@ uint32_t default_unit(uint64_t items, uint64_t positions, uint32_t large) {
   return u;
 }
 
+// A stage group: the same transform (log_n, direction, coset) applied to
+// columns of several jobs, planned and run as one batch through per-column
+// pointer tables.
+struct StageKey {
+  uint32_t log_n, bitrev, inv, ext, coset;
+  bool operator<(const StageKey &o) const {
+    if (log_n != o.log_n) return log_n < o.log_n;
+    if (ext != o.ext) return ext < o.ext;  // inverse-direction stages (ext 0) first
+    if (bitrev != o.bitrev) return bitrev < o.bitrev;
+    if (inv != o.inv) return inv < o.inv;
+    return coset < o.coset;
+  }
+};
+
+struct StageMember {
+  const uint64_t *in;
+  uint64_t *out;
+  uint64_t si, so;
+  uint32_t cols;
+};
+
 struct PlanSeg {
   RaggedSeg s;
-  uint32_t level, job, stage, pass;
-  int32_t dep_key;  // index into the unsorted list
+  uint32_t level, group;
 };
 
 }  // namespace
@@ -213,118 +234,113 @@ int ragged_validate_job(const bc_ragged_job &j) {
 }
 
 int ragged_build(const bc_ragged_job *jobs, uint32_t num_jobs, const Tables &tab, RaggedHostPlan &out) {
-  out.segs.clear();
-  out.total_items = 0;
-  out.levels = 0;
+  out = RaggedHostPlan{};
   for (uint32_t j = 0; j < num_jobs; ++j) {
     const int rc = ragged_validate_job(jobs[j]);
     if (rc != BC_SUCCESS) return rc;
   }
-  std::vector<PlanSeg> segs;
+  // stages of every job, grouped by transform: an NTT job is one forward
+  // natural->bit-reversed stage, an iNTT job one inverse bit-reversed->natural
+  // stage, an LDE job that inverse stage (into coeffs) plus one forward coset
+  // stage per coset (coeffs -> output block brev(c))
+  std::map<StageKey, std::vector<StageMember>> groups;
   for (uint32_t j = 0; j < num_jobs; ++j) {
     const bc_ragged_job &jb = jobs[j];
     if (jb.num_cols == 0) continue;
-    struct Stage {
-      const uint64_t *in;
-      uint64_t *out;
-      uint64_t si, so;
-      bool bitrev, inv;
-      uint32_t ext, coset;
-    };
-    std::vector<Stage> stages;
     const uint64_t n = 1ull << jb.log_n;
-    if (jb.op == BC_OP_NTT) stages.push_back({jb.inputs, jb.outputs, jb.stride_in, jb.stride_out, false, false, 0, 0});
-    else if (jb.op == BC_OP_INTT) stages.push_back({jb.inputs, jb.outputs, jb.stride_in, jb.stride_out, true, true, 0, 0});
-    else {
-      stages.push_back({jb.inputs, jb.coeffs, jb.stride_in, jb.stride_coeffs, true, true, 0, 0});
+    if (jb.op == BC_OP_NTT) {
+      groups[{jb.log_n, 0, 0, 0, 0}].push_back({jb.inputs, jb.outputs, jb.stride_in, jb.stride_out, jb.num_cols});
+    } else if (jb.op == BC_OP_INTT) {
+      groups[{jb.log_n, 1, 1, 0, 0}].push_back({jb.inputs, jb.outputs, jb.stride_in, jb.stride_out, jb.num_cols});
+    } else {
+      groups[{jb.log_n, 1, 1, 0, 0}].push_back({jb.inputs, jb.coeffs, jb.stride_in, jb.stride_coeffs, jb.num_cols});
       for (uint32_t c = 0; c < (1u << jb.log_blowup); ++c)
-        stages.push_back({jb.coeffs, jb.outputs + (uint64_t)brev_bits_host(c, jb.log_blowup) * n, jb.stride_coeffs,
-                          jb.stride_out, false, false, jb.log_blowup, c});
-    }
-    const TuneParams tp = tuned(jb.log_n, jb.num_cols);
-    int32_t stage0_last = -1;
-    uint32_t stage0_passes = 0;
-    for (uint32_t st = 0; st < stages.size(); ++st) {
-      const Stage &sgd = stages[st];
-      PassDesc pd[8];
-      const int np = describe_fast_passes(tab, sgd.in, sgd.out, jb.log_n, jb.num_cols, sgd.si, sgd.so, sgd.bitrev,
-                                          sgd.inv, sgd.ext, sgd.coset, pd);
-      if (np <= 0) return BC_ERROR_INVALID_VALUE;
-      const uint32_t base_level = st == 0 ? 0 : stage0_passes;
-      int32_t prev = st == 0 ? -1 : stage0_last;
-      for (int k = 0; k < np; ++k) {
-        PlanSeg ps{};
-        ps.level = base_level + k;
-        ps.job = j;
-        ps.stage = st;
-        ps.pass = k;
-        ps.dep_key = prev;
-        ps.s.a = pd[k].a;
-        uint32_t dm;
-        if (!sgd.inv && !sgd.bitrev) dm = pd[k].mode == MODE_LDE_IN ? 1u : 0u;
-        else if (sgd.inv && sgd.bitrev) dm = pd[k].mode == MODE_NINV ? 3u : 2u;
-        else return BC_ERROR_INVALID_VALUE;
-        if ((dm == 0 || dm == 2) && pd[k].mode != MODE_NONE) return BC_ERROR_INVALID_VALUE;
-        if (pd[k].contig) {
-          const bool mc = jb.log_n < TILE_LOG;
-          ps.s.variant = variant_id(false, pd[k].log_h, mc, dm);
-          ps.s.total = pd[k].a.tiles;
-          ps.s.per_unit = tp.unit_contig ? tp.unit_contig : default_unit(ps.s.total, 1, 4);
-          ps.s.units = (uint32_t)((ps.s.total + ps.s.per_unit - 1) / ps.s.per_unit);
-          ps.s.positions = 0;
-        } else {
-          ps.s.variant = variant_id(true, pd[k].log_h, false, dm);
-          ps.s.total = pd[k].pairs;
-          ps.s.per_unit = (uint32_t)std::min<uint64_t>(
-              pd[k].pairs, tp.unit_strided ? tp.unit_strided : default_unit(pd[k].pairs, pd[k].positions, 8));
-          ps.s.positions = (uint32_t)pd[k].positions;
-          const uint64_t groups = (ps.s.total + ps.s.per_unit - 1) / ps.s.per_unit;
-          ps.s.units = (uint32_t)(groups * ps.s.positions);
-        }
-        prev = (int32_t)segs.size();
-        segs.push_back(ps);
-      }
-      if (st == 0) {
-        stage0_last = prev;
-        stage0_passes = (uint32_t)np;
-      }
+        groups[{jb.log_n, 0, 0, jb.log_blowup, c}].push_back(
+            {jb.coeffs, jb.outputs + (uint64_t)brev_bits_host(c, jb.log_blowup) * n, jb.stride_coeffs, jb.stride_out,
+             jb.num_cols});
     }
   }
+  // inverse stages (ext 0) are keyed before the coset stages of the same
+  // log_n, so their pass counts are known when a coset stage needs its level
+  std::map<uint32_t, uint32_t> inverse_passes;  // log_n -> passes of its inverse group
+  std::vector<PlanSeg> segs;
+  uint32_t gi = 0;
+  for (const auto &kv : groups) {
+    const StageKey &key = kv.first;
+    uint64_t total_cols = 0;
+    for (const StageMember &m : kv.second) total_cols += m.cols;
+    if (total_cols > 0xFFFFFFFFull) return BC_ERROR_INVALID_VALUE;
+    PassDesc pd[8];
+    const uint64_t n = 1ull << key.log_n;
+    const int np = describe_fast_passes(tab, nullptr, nullptr, key.log_n, (uint32_t)total_cols, n, n, key.bitrev != 0,
+                                        key.inv != 0, key.ext, key.coset, pd);
+    if (np <= 0) return BC_ERROR_INVALID_VALUE;
+    uint32_t base_level = 0;
+    if (key.ext) {
+      auto it = inverse_passes.find(key.log_n);
+      if (it == inverse_passes.end()) return BC_ERROR_INVALID_VALUE;
+      base_level = it->second;
+    } else if (key.inv) {
+      inverse_passes[key.log_n] = (uint32_t)np;
+    }
+    const TuneParams tp = tuned(key.log_n, (uint32_t)total_cols);
+    for (int k = 0; k < np; ++k) {
+      PlanSeg ps{};
+      ps.level = base_level + (uint32_t)k;
+      ps.group = gi;
+      ps.s.a = pd[k].a;
+      ps.s.a.in = nullptr;
+      ps.s.a.out = nullptr;
+      // column table of this pass: the first pass reads the stage's inputs,
+      // later passes work in place on its outputs
+      ps.s.col_off = out.cols.size();
+      for (const StageMember &m : kv.second)
+        for (uint32_t c = 0; c < m.cols; ++c)
+          out.cols.push_back({k == 0 ? m.in + c * m.si : m.out + c * m.so, m.out + c * m.so});
+      uint32_t dm;
+      if (!key.inv && !key.bitrev) dm = pd[k].mode == MODE_LDE_IN ? 1u : 0u;
+      else if (key.inv && key.bitrev) dm = pd[k].mode == MODE_NINV ? 3u : 2u;
+      else return BC_ERROR_INVALID_VALUE;
+      if ((dm == 0 || dm == 2) && pd[k].mode != MODE_NONE) return BC_ERROR_INVALID_VALUE;
+      if (pd[k].contig) {
+        const bool mc = key.log_n < TILE_LOG;
+        ps.s.variant = variant_id(false, pd[k].log_h, mc, dm);
+        ps.s.total = pd[k].a.tiles;
+        ps.s.per_unit = tp.unit_contig ? tp.unit_contig : default_unit(ps.s.total, 1, 4);
+        ps.s.units = (uint32_t)((ps.s.total + ps.s.per_unit - 1) / ps.s.per_unit);
+        ps.s.positions = 0;
+      } else {
+        ps.s.variant = variant_id(true, pd[k].log_h, false, dm);
+        ps.s.total = pd[k].pairs;
+        ps.s.per_unit = (uint32_t)std::min<uint64_t>(
+            pd[k].pairs, tp.unit_strided ? tp.unit_strided : default_unit(pd[k].pairs, pd[k].positions, 8));
+        ps.s.positions = (uint32_t)pd[k].positions;
+        const uint64_t ngroups = (ps.s.total + ps.s.per_unit - 1) / ps.s.per_unit;
+        if (ngroups * ps.s.positions > 0xFFFFFFFFull) return BC_ERROR_INVALID_VALUE;
+        ps.s.units = (uint32_t)(ngroups * ps.s.positions);
+      }
+      segs.push_back(ps);
+    }
+    ++gi;
+  }
 
-  // order: level, then variant, then job/stage; remap dependencies
-  std::vector<uint32_t> order(segs.size());
-  for (uint32_t i = 0; i < order.size(); ++i) order[i] = i;
-  std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
-    const PlanSeg &a = segs[x], &b = segs[y];
+  // queue order: level, then kernel variant, then group
+  std::stable_sort(segs.begin(), segs.end(), [](const PlanSeg &a, const PlanSeg &b) {
     if (a.level != b.level) return a.level < b.level;
     if (a.s.variant != b.s.variant) return a.s.variant < b.s.variant;
-    if (a.job != b.job) return a.job < b.job;
-    return a.stage < b.stage;
+    return a.group < b.group;
   });
-  std::vector<uint32_t> pos_of(segs.size());
-  for (uint32_t i = 0; i < order.size(); ++i) pos_of[order[i]] = i;
   out.segs.resize(segs.size());
-  out.level_seg.clear();
-  out.level_item.clear();
   uint64_t item = 0;
-  for (uint32_t i = 0; i < order.size(); ++i) {
-    const PlanSeg &ps = segs[order[i]];
-    RaggedSeg r = ps.s;
+  for (uint32_t i = 0; i < segs.size(); ++i) {
+    RaggedSeg r = segs[i].s;
     r.first_item = item;
     item += r.units;
-    if (ps.dep_key >= 0) {
-      r.dep = (int32_t)pos_of[ps.dep_key];
-      r.dep_units = segs[ps.dep_key].s.units;
-      if (segs[ps.dep_key].level >= ps.level) return BC_ERROR_INVALID_VALUE;  // levels order the launches
-    } else {
-      r.dep = -1;
-      r.dep_units = 0;
-    }
-    while (out.level_seg.size() <= ps.level) {  // first segment of each level
+    while (out.level_seg.size() <= segs[i].level) {  // first segment of each level
       out.level_seg.push_back(i);
       out.level_item.push_back(r.first_item);
     }
-    out.levels = std::max(out.levels, ps.level + 1);
+    out.levels = std::max(out.levels, segs[i].level + 1);
     out.segs[i] = r;
   }
   if (out.levels > RAGGED_MAX_LEVELS) return BC_ERROR_INVALID_VALUE;
@@ -345,15 +361,26 @@ int ragged_build(const bc_ragged_job *jobs, uint32_t num_jobs, const Tables &tab
   return BC_SUCCESS;
 }
 
-size_t ragged_table_bytes(const RaggedHostPlan &hp) {
-  return std::max<size_t>(1, hp.segs.size()) * sizeof(RaggedSeg) + hp.chunk_seg.size() * sizeof(uint32_t);
+namespace {
+// byte offsets of the packed table: segments, chunk index, column tables
+size_t chunk_offset(const RaggedHostPlan &hp) { return std::max<size_t>(1, hp.segs.size()) * sizeof(RaggedSeg); }
+size_t cols_offset(const RaggedHostPlan &hp) {
+  return (chunk_offset(hp) + hp.chunk_seg.size() * sizeof(uint32_t) + 15) / 16 * 16;
 }
+}  // namespace
 
-void ragged_pack_table(const RaggedHostPlan &hp, void *host_buf) {
+size_t ragged_table_bytes(const RaggedHostPlan &hp) { return cols_offset(hp) + hp.cols.size() * sizeof(ColPtrs); }
+
+void ragged_pack_table(const RaggedHostPlan &hp, void *host_buf, const void *device_buf) {
   uint8_t *b = (uint8_t *)host_buf;
-  if (!hp.segs.empty()) memcpy(b, hp.segs.data(), hp.segs.size() * sizeof(RaggedSeg));
-  memcpy(b + std::max<size_t>(1, hp.segs.size()) * sizeof(RaggedSeg), hp.chunk_seg.data(),
-         hp.chunk_seg.size() * sizeof(uint32_t));
+  const ColPtrs *dev_cols = (const ColPtrs *)((const uint8_t *)device_buf + cols_offset(hp));
+  RaggedSeg *segs = (RaggedSeg *)b;
+  for (size_t i = 0; i < hp.segs.size(); ++i) {
+    segs[i] = hp.segs[i];
+    segs[i].a.cols = dev_cols + hp.segs[i].col_off;
+  }
+  memcpy(b + chunk_offset(hp), hp.chunk_seg.data(), hp.chunk_seg.size() * sizeof(uint32_t));
+  if (!hp.cols.empty()) memcpy(b + cols_offset(hp), hp.cols.data(), hp.cols.size() * sizeof(ColPtrs));
 }
 
 size_t ragged_state_bytes() { return sizeof(QueueState); }
@@ -370,8 +397,7 @@ hipError_t ragged_launch(const void *d_table, const RaggedHostPlan &hp, void *d_
                          hipStream_t s) {
   QueueState *st = (QueueState *)d_state;
   const RaggedSeg *d_segs = (const RaggedSeg *)d_table;
-  const uint32_t *d_chunks = (const uint32_t *)((const uint8_t *)d_table +
-                                                std::max<size_t>(1, hp.segs.size()) * sizeof(RaggedSeg));
+  const uint32_t *d_chunks = (const uint32_t *)((const uint8_t *)d_table + chunk_offset(hp));
   ragged_reset_kernel<<<dim3(1), dim3(64), 0, s>>>(st);
   hipError_t err = hipGetLastError();
   for (uint32_t l = 0; l < hp.levels && err == hipSuccess; ++l) {
@@ -424,10 +450,12 @@ extern "C" int bc_ragged_plan_create(const bc_ragged_job *jobs, uint32_t num_job
   p->workgroups = ragged_workgroups(dev);
   const size_t table_bytes = ragged_table_bytes(hp);
   std::vector<uint64_t> packed((table_bytes + 7) / 8);
-  ragged_pack_table(hp, packed.data());
   err = hipMalloc(&p->d_table, table_bytes);
   if (err == hipSuccess) err = hipMalloc(&p->d_state, ragged_state_bytes());
-  if (err == hipSuccess) err = hipMemcpy(p->d_table, packed.data(), table_bytes, hipMemcpyHostToDevice);
+  if (err == hipSuccess) {
+    ragged_pack_table(hp, packed.data(), p->d_table);
+    err = hipMemcpy(p->d_table, packed.data(), table_bytes, hipMemcpyHostToDevice);
+  }
   p->hp = std::move(hp);
   if (err != hipSuccess) {
     (void)hipFree(p->d_table);

@@ -15,10 +15,9 @@ struct RaggedSeg {
   uint64_t total;       // contiguous: tiles; strided: (column, block) pairs
   uint32_t units;
   uint32_t variant;
-  int32_t dep;          // segment that must be complete first, -1 if none (planning check;
-  uint32_t dep_units;   // the level launches enforce the order on the device)
   uint32_t per_unit;    // contiguous: tiles per unit; strided: pairs per unit
   uint32_t positions;   // strided: r-ranges
+  uint64_t col_off;     // first entry of this segment's column table (a.cols is set when packed)
 };
 
 // Dependency levels of a plan: pass k of a job's first stage is level k; the
@@ -33,6 +32,7 @@ struct RaggedHostPlan {
   std::vector<uint32_t> level_seg;    // levels + 1 entries: first segment of each level, then segs.size()
   std::vector<uint64_t> level_item;   // levels + 1 entries: first queue item of each level, then total_items
   std::vector<uint32_t> chunk_seg;    // entry c: segment holding item c << RAGGED_CHUNK_LOG; last entry: segs.size() - 1
+  std::vector<ColPtrs> cols;          // column tables of all segments
   uint64_t total_items = 0;
   uint32_t levels = 0;
 };
@@ -44,10 +44,12 @@ int ragged_build(const bc_ragged_job *jobs, uint32_t num_jobs, const Tables &tab
 size_t ragged_state_bytes();
 // Persistent workgroups that fit on device `dev` at once.
 uint32_t ragged_workgroups(int dev);
-// Device image of a plan: the segment table followed by the chunk index, in
-// one buffer of ragged_table_bytes(hp) bytes (8-byte aligned).
+// Device image of a plan: the segment table, the chunk index and the column
+// tables, in one buffer of ragged_table_bytes(hp) bytes.  Packing writes the
+// device addresses of the column tables, so it needs the buffer's device
+// address (device_buf) as well as the host staging copy.
 size_t ragged_table_bytes(const RaggedHostPlan &hp);
-void ragged_pack_table(const RaggedHostPlan &hp, void *host_buf);
+void ragged_pack_table(const RaggedHostPlan &hp, void *host_buf, const void *device_buf);
 // Resets the queue state and launches the persistent kernel once per
 // dependency level on `s`; the packed table must already be in device memory
 // at d_table.  Allocates nothing.

@@ -73,8 +73,11 @@ int64_t now_ns() {
 
 constexpr uint64_t ARENA_ALIGN = 64 * 1024;
 constexpr uint32_t SLOT_MAX_SEGS = 8192;
-// segment table plus a chunk index of up to 2^18 entries (2^24 queue items)
-constexpr size_t SLOT_TABLE_BYTES = SLOT_MAX_SEGS * sizeof(RaggedSeg) + (1u << 18) * sizeof(uint32_t);
+constexpr uint64_t SLOT_COL_CAP = 1u << 19;  // column-table entries per batch
+// segment table, a chunk index of up to 2^18 entries (2^24 queue items) and
+// the column tables
+constexpr size_t SLOT_TABLE_BYTES = SLOT_MAX_SEGS * sizeof(RaggedSeg) + (1u << 18) * sizeof(uint32_t) + 16 +
+                                    SLOT_COL_CAP * sizeof(ColPtrs);
 constexpr int32_t STATUS_QUEUE_ERROR = 702;  // hipErrorLaunchTimeOut: a unit gave up waiting
 
 struct Request;
@@ -86,6 +89,7 @@ struct Part {
   uint64_t out_off = 0, tmp_off = 0;
   uint64_t work = 0;                      // transformed elements
   uint32_t seg_bound = 0;                 // upper bound on its ragged segments
+  uint64_t col_bound = 0;                 // upper bound on its column-table entries
   bool out_held = false;                  // outputs still allocated in the arena
 };
 
@@ -170,15 +174,18 @@ void bc_server::finish_parts(std::vector<bc::Part *> &parts, int32_t err) {
 bool bc_server::launch_batch(bc::Slot &sl) {
   using namespace bc;
   std::vector<Part *> parts;
-  uint64_t work = 0, segs_bound = 0;
+  uint64_t work = 0, segs_bound = 0, cols_bound = 0;
   const uint64_t work_cap = 2 * cfg.max_part_elems;
   while (!admitted.empty() && parts.size() < cfg.max_parts_per_batch) {
     Part *p = admitted.front();
-    if (!parts.empty() && (work + p->work > work_cap || segs_bound + p->seg_bound > SLOT_MAX_SEGS)) break;
+    if (!parts.empty() && (work + p->work > work_cap || segs_bound + p->seg_bound > SLOT_MAX_SEGS ||
+                           cols_bound + p->col_bound > SLOT_COL_CAP))
+      break;
     parts.push_back(p);
     admitted.pop_front();
     work += p->work;
     segs_bound += p->seg_bound;
+    cols_bound += p->col_bound;
   }
   if (parts.empty()) return false;
 
@@ -222,7 +229,7 @@ bool bc_server::launch_batch(bc::Slot &sl) {
     return true;
   }
   const size_t table_bytes = ragged_table_bytes(hp);
-  ragged_pack_table(hp, sl.h_table);
+  ragged_pack_table(hp, sl.h_table, sl.d_table);
   hipError_t err = hipMemcpyAsync(sl.d_table, sl.h_table, table_bytes, hipMemcpyHostToDevice, sl.stream);
   for (size_t k = 0; k < parts.size() && err == hipSuccess; ++k) {
     const Part &p = *parts[k];
@@ -373,6 +380,8 @@ extern "C" int bc_server_submit(bc_server *s, const bc_request *rq, uint64_t *id
   const uint64_t cap = ar.capacity();
   if (ar.round(out_col) + ar.round(tmp_col) > cap) return hipErrorOutOfMemory;
   uint64_t cols_pp = std::max<uint64_t>(1, std::min<uint64_t>(r.num_cols, s->cfg.max_part_elems / work_col));
+  // column tables: at most 8 passes per stage, and room for two parts per batch
+  cols_pp = std::max<uint64_t>(1, std::min<uint64_t>(cols_pp, SLOT_COL_CAP / (2 * 8 * stages)));
   while (cols_pp > 1 && ar.round(out_col * cols_pp) + ar.round(tmp_col * cols_pp) > cap) cols_pp /= 2;
   const uint32_t nparts = (uint32_t)((r.num_cols + cols_pp - 1) / cols_pp);
 
@@ -390,6 +399,7 @@ extern "C" int bc_server_submit(bc_server *s, const bc_request *rq, uint64_t *id
     p.tmp_bytes = tmp_col ? ar.round(tmp_col * p.cols) : 0;
     p.work = work_col * p.cols;
     p.seg_bound = (uint32_t)(8 * stages);
+    p.col_bound = 8 * stages * p.cols;
     kept += p.out_bytes;
     max_tmp = std::max(max_tmp, p.tmp_bytes);
   }
