@@ -178,9 +178,10 @@ uint32_t brev_bits_host(uint32_t x, uint32_t bits) {
 }
 
 // Work-unit size when none is tuned: `large` items (tiles or pairs) per unit,
-// halved while the segment would have fewer than 64 units (items / size units
-// at each of `positions` r-ranges), so that small segments still spread over
-// many workgroups.
+// which amortizes a unit's setup (twists, coset factors, queue pop) the way a
+// standalone workgroup does, halved while the segment would have fewer than 64
+// units (items / size units at each of `positions` r-ranges), so that small
+// segments still spread over many workgroups.
 uint32_t default_unit(uint64_t items, uint64_t positions, uint32_t large) {
   uint32_t u = large;
   while (u > 1 && ((items + u - 1) / u) * positions < 64) u /= 2;
@@ -306,14 +307,14 @@ int ragged_build(const bc_ragged_job *jobs, uint32_t num_jobs, const Tables &tab
         const bool mc = key.log_n < TILE_LOG;
         ps.s.variant = variant_id(false, pd[k].log_h, mc, dm);
         ps.s.total = pd[k].a.tiles;
-        ps.s.per_unit = tp.unit_contig ? tp.unit_contig : default_unit(ps.s.total, 1, 4);
+        ps.s.per_unit = tp.unit_contig ? tp.unit_contig : default_unit(ps.s.total, 1, 32);
         ps.s.units = (uint32_t)((ps.s.total + ps.s.per_unit - 1) / ps.s.per_unit);
         ps.s.positions = 0;
       } else {
         ps.s.variant = variant_id(true, pd[k].log_h, false, dm);
         ps.s.total = pd[k].pairs;
         ps.s.per_unit = (uint32_t)std::min<uint64_t>(
-            pd[k].pairs, tp.unit_strided ? tp.unit_strided : default_unit(pd[k].pairs, pd[k].positions, 8));
+            pd[k].pairs, tp.unit_strided ? tp.unit_strided : default_unit(pd[k].pairs, pd[k].positions, 32));
         ps.s.positions = (uint32_t)pd[k].positions;
         const uint64_t ngroups = (ps.s.total + ps.s.per_unit - 1) / ps.s.per_unit;
         if (ngroups * ps.s.positions > 0xFFFFFFFFull) return BC_ERROR_INVALID_VALUE;
