@@ -40,10 +40,15 @@ GL_FN uint64_t add(uint64_t a, uint64_t b) {
   return ((s < a) | (t < s)) ? t : s;
 }
 
-// a - b mod p for CANONICAL a, b; result canonical.
+// a - b mod p for CANONICAL a, b; result canonical.  The borrow comes from
+// the 32-bit subtract chain itself (no separate comparison); on a borrow
+// a - b + 2^64 >= 2^64 - p + 1, so adding p (mod 2^64) lands in [1, p).
 GL_FN uint64_t sub(uint64_t a, uint64_t b) {
-  const uint64_t d = a - b;
-  return a < b ? d + P : d;
+  unsigned b0, b1;
+  const uint32_t lo = __builtin_subc((uint32_t)a, (uint32_t)b, 0u, &b0);
+  const uint32_t hi = __builtin_subc((uint32_t)(a >> 32), (uint32_t)(b >> 32), b0, &b1);
+  const uint64_t d = ((uint64_t)hi << 32) | lo;
+  return b1 ? d + P : d;
 }
 
 GL_FN uint64_t neg(uint64_t a) { return sub(0ull, a); }
@@ -71,19 +76,21 @@ GL_FN uint64_t reduce_limbs(uint32_t l0, uint32_t l1, uint32_t l2, uint32_t l3) 
   return fold(((uint64_t)hi << 32) | lo, c && !b1, b1 && !c);
 }
 
-// a * b mod p for any u64 a, b; result canonical.  The 128-bit product takes
-// four 32x32+64 multiply-adds: the middle column's addend (p00_hi, p11_lo)
-// already has the right weight, and its carries land in the top limb.
+// a * b mod p for any u64 a, b; result canonical.  The 128-bit product is
+// accumulated column by column through the 64-bit addend of four
+// 32x32+64 multiply-adds; every partial sum stays below 2^64
+// ((2^32 - 1)^2 + 2 (2^32 - 1) = 2^64 - 1), so no carry needs checking:
+//   t1 = a0 b1 + hi(a0 b0),  t2 = a1 b0 + lo(t1),  t3 = a1 b1 + hi(t1),
+//   t4 = t3 + hi(t2);  limbs lo(a0 b0), lo(t2), lo(t4), hi(t4).
 GL_FN uint64_t mul(uint64_t a, uint64_t b) {
   const uint32_t a0 = (uint32_t)a, a1 = (uint32_t)(a >> 32);
   const uint32_t b0 = (uint32_t)b, b1 = (uint32_t)(b >> 32);
-  const uint64_t p00 = (uint64_t)a0 * b0;
-  const uint64_t p11 = (uint64_t)a1 * b1;
-  uint64_t m;
-  const bool c1 = __builtin_add_overflow((uint64_t)a0 * b1, (p11 << 32) | (p00 >> 32), &m);
-  const bool c2 = __builtin_add_overflow((uint64_t)a1 * b0, m, &m);
-  const uint32_t l3 = (uint32_t)(p11 >> 32) + (uint32_t)c1 + (uint32_t)c2;  // exact: product < 2^128
-  return reduce_limbs((uint32_t)p00, (uint32_t)m, (uint32_t)(m >> 32), l3);
+  const uint64_t t0 = (uint64_t)a0 * b0;
+  const uint64_t t1 = (uint64_t)a0 * b1 + (t0 >> 32);
+  const uint64_t t2 = (uint64_t)a1 * b0 + (uint32_t)t1;
+  const uint64_t t3 = (uint64_t)a1 * b1 + (t1 >> 32);
+  const uint64_t t4 = t3 + (t2 >> 32);
+  return reduce_limbs((uint32_t)t0, (uint32_t)t2, (uint32_t)t4, (uint32_t)(t4 >> 32));
 }
 
 // x * 2^E for 0 < E < 64, canonical result, any u64 x.
