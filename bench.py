@@ -198,6 +198,45 @@ def bench_serve(count, rate, budget_bytes, seed=5):
                          "frac": gbs / HBM_PEAK_GBS}}
 
 
+def bench_ops(log_n, cols, reps):
+    """Order conversion and domain powers around the NTT on the batched
+    shape: bit_reverse (out of place and in place, 16 B per element moved),
+    omega_shift (16 B per element) and get_powers_of_w (8 B per element
+    written).  HBM-bound kernels: GB/s against the 8 TB/s peak."""
+    import torch
+    from boojum_cuda_amd import ops
+    n = 1 << log_n
+    g = torch.Generator(device="cuda")
+    g.manual_seed(6)
+    x = uniform_field(cols * n, g)
+    y = torch.empty_like(x)
+    stream = torch.cuda.current_stream()
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2 * reps)]
+        for r in range(reps):
+            ev[2 * r].record(stream)
+            fn()
+            ev[2 * r + 1].record(stream)
+        torch.cuda.synchronize()
+        return sum(ev[2 * r].elapsed_time(ev[2 * r + 1]) for r in range(reps)) / reps
+
+    total = cols * n
+    out = {}
+    for name, fn, bytes_per in [
+            ("bit_reverse", lambda: ops.bit_reverse(x, y, n, cols), 16),
+            ("bit_reverse_in_place", lambda: ops.bit_reverse_in_place(x, n, cols), 16),
+            ("omega_shift", lambda: ops.omega_shift(x, log_n + 8, 0, False, 3, y), 16),
+            ("get_powers_of_w", lambda: ops.get_powers_of_w(log_n + 8, 0, False, False, y), 8)]:
+        ms = timed(fn)
+        gbs = bytes_per * total / (ms * 1e-3) / 1e9
+        out[name] = {"ms": ms, "GB/s": gbs, "frac": gbs / HBM_PEAK_GBS}
+    del x, y
+    return {"workload": f"{cols} x 2^{log_n} u64 ({total} elements)", **out}
+
+
 def cpu_baseline(log_n, cols, threads):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import numpy as np
@@ -242,6 +281,7 @@ def main():
     ap.add_argument("--no-ragged", action="store_true", help="skip the ragged-batch measurement")
     ap.add_argument("--ragged-jobs", type=int, default=1000)
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop measurement")
+    ap.add_argument("--no-ops", action="store_true", help="skip the bit-reverse / powers / omega-shift measurement")
     ap.add_argument("--serve-requests", type=int, default=100)
     ap.add_argument("--serve-rate", type=float, default=25.0, help="offered load, requests per second")
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
@@ -318,6 +358,10 @@ def main():
     if not args.no_ragged:
         extras["ragged"] = bench_ragged(args.ragged_jobs, 10, 23, 5, 1)
         progress(f"ragged done: {extras['ragged']['ms_ragged']:.2f} ms vs plain {extras['ragged']['ms_plain']:.2f} ms")
+    if not args.no_ops:
+        torch.cuda.empty_cache()
+        extras["ops"] = bench_ops(log_n, cols, 5)
+        progress(f"ops done: bit_reverse {extras['ops']['bit_reverse']['GB/s']:.0f} GB/s")
     if not args.no_serve:
         torch.cuda.empty_cache()
         extras["serve"] = bench_serve(args.serve_requests, args.serve_rate, int(args.serve_budget_gb * 1e9))

@@ -39,6 +39,12 @@ SIGNATURES = {
     "bc_tune_clear": (c_int, []),
     "bc_tune_load": (c_int, [ctypes.c_char_p]),
     "bc_tune_save": (c_int, [ctypes.c_char_p]),
+    "bc_bit_reverse": (c_int, [c_vp, c_u64, c_vp, c_u64, c_u64, c_u32, c_vp]),
+    "bc_bit_reverse_in_place": (c_int, [c_vp, c_u64, c_u64, c_u32, c_vp]),
+    "bc_get_powers_of_w": (c_int, [c_u32, c_u32, c_bool, c_bool, c_vp, c_u32, c_vp]),
+    "bc_get_powers_of_g": (c_int, [c_u32, c_u32, c_bool, c_bool, c_vp, c_u32, c_vp]),
+    "bc_omega_shift": (c_int, [c_vp, c_u32, c_u32, c_bool, c_u32, c_vp, c_u32, c_vp]),
+    "bc_omega_shift_in_place": (c_int, [c_vp, c_u32, c_u32, c_bool, c_u32, c_u32, c_vp]),
     "bc_fill_seeded": (c_int, [c_vp, c_u32, c_u32, c_u64, c_u64, c_u32, c_vp]),
     "bc_server_create": (c_int, [c_vp, ctypes.POINTER(c_vp)]),
     "bc_server_submit": (c_int, [c_vp, c_vp, ctypes.POINTER(c_u64)]),

@@ -153,6 +153,35 @@ int bc_ragged_plan_info(const bc_ragged_plan *plan, uint64_t *units, uint32_t *l
  * (one bc_batch_ntt_internal / bc_batch_lde per job). */
 int bc_ragged_run_plain(const bc_ragged_job *jobs, uint32_t num_jobs, void *stream);
 
+/* ---- Order conversion and domain powers around the NTT ---- */
+
+/* bit_reverse (reference: src/ops_complex.rs:260-318): for each of `cols`
+ * columns (stride apart), dst[brev(i)] = src[i], i < rows; values are copied
+ * unchanged.  rows must be a power of two below 2^32 (src/ops_complex.rs:279-
+ * 281).  src == dst with equal strides is the in-place permutation. */
+int bc_bit_reverse(const uint64_t *src, uint64_t src_stride, uint64_t *dst, uint64_t dst_stride, uint64_t rows,
+                   uint32_t cols, void *stream);
+/* bit_reverse_in_place (reference: src/ops_complex.rs:320-330). */
+int bc_bit_reverse_in_place(uint64_t *values, uint64_t stride, uint64_t rows, uint32_t cols, void *stream);
+
+/* get_powers_of_w / get_powers_of_g (reference: src/ops_complex.rs:29-86,
+ * native/ops_complex.cu:9-33): result[i] = omega_{2^log_degree}^(+-idx), or
+ * g^(+-((idx << (24 - log_degree)) mod 2^24)) (the reference's 24-bit g
+ * tables: log_degree <= 24), with idx = offset + i, bit-reversed over
+ * log_degree bits when bit_reverse.  Needs a context. */
+int bc_get_powers_of_w(uint32_t log_degree, uint32_t offset, bool inverse, bool bit_reverse, uint64_t *result,
+                       uint32_t count, void *stream);
+int bc_get_powers_of_g(uint32_t log_degree, uint32_t offset, bool inverse, bool bit_reverse, uint64_t *result,
+                       uint32_t count, void *stream);
+
+/* omega_shift / omega_shift_in_place (reference: src/ops_complex.rs:207-258,
+ * native/ops_complex.cu:76-87): dst[i] = src[i] * omega_{2^log_degree}^(+-
+ * shift * (offset + i)), canonical.  Needs a context. */
+int bc_omega_shift(const uint64_t *src, uint32_t log_degree, uint32_t offset, bool inverse, uint32_t shift,
+                   uint64_t *dst, uint32_t count, void *stream);
+int bc_omega_shift_in_place(uint64_t *values, uint32_t log_degree, uint32_t offset, bool inverse, uint32_t shift,
+                            uint32_t count, void *stream);
+
 /* Seeded inputs on the device: column c (c = 0..num_cols-1) at
  * out + c * stride gets 2^log_n u64 draws of the splitmix64 stream seeded with
  * seed * 0x9E3779B97F4A7C15 + first_col + c (raw u64, any value, read mod p).
