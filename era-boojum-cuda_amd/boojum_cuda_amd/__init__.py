@@ -1,7 +1,7 @@
 """MI355X (gfx950) batched Goldilocks NTT / coset-LDE, a drop-in for
 boojum-cuda's `ntt` and `context` modules.  Compute runs in the HIP library
 libboojum_ntt.so (C ABI: include/boojum_ntt.h)."""
-from . import context, lde, ntt, ops, ragged, serve  # noqa: F401
+from . import barycentric, context, lde, ntt, ops, ragged, serve  # noqa: F401
 from ._lib import HipError, LIB_PATH  # noqa: F401
 
 P = 0xFFFFFFFF00000001

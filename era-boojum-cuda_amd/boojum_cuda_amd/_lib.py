@@ -45,6 +45,9 @@ SIGNATURES = {
     "bc_get_powers_of_g": (c_int, [c_u32, c_u32, c_bool, c_bool, c_vp, c_u32, c_vp]),
     "bc_omega_shift": (c_int, [c_vp, c_u32, c_u32, c_bool, c_u32, c_vp, c_u32, c_vp]),
     "bc_omega_shift_in_place": (c_int, [c_vp, c_u32, c_u32, c_bool, c_u32, c_u32, c_vp]),
+    "bc_barycentric_precompute": (c_int, [c_bool, c_vp, c_vp, c_u64, c_vp, c_u64, c_bool, c_vp]),
+    "bc_batch_eval_temp_sizes": (c_int, [c_u32, c_u64, c_u32, ctypes.POINTER(c_size), ctypes.POINTER(c_size)]),
+    "bc_batch_eval": (c_int, [c_u32, c_vp, c_u64, c_u32, c_vp, c_vp, c_size, c_vp, c_size, c_vp, c_vp]),
     "bc_fill_seeded": (c_int, [c_vp, c_u32, c_u32, c_u64, c_u64, c_u32, c_vp]),
     "bc_server_create": (c_int, [c_vp, ctypes.POINTER(c_vp)]),
     "bc_server_submit": (c_int, [c_vp, c_vp, ctypes.POINTER(c_u64)]),

@@ -93,6 +93,38 @@ GL_FN uint64_t mul(uint64_t a, uint64_t b) {
   return reduce_limbs((uint32_t)t0, (uint32_t)t2, (uint32_t)t4, (uint32_t)(t4 >> 32));
 }
 
+// Sum of products kept unreduced in five 32-bit limbs,
+// l0 + l1 2^32 + l2 2^64 + l3 2^96 + l4 2^128: each product costs the four
+// multiply-adds of mul plus a 5-step carry chain, and the reduction is paid
+// once per sum.  l4 counts carries out of 128 bits, so it stays below the
+// number of products (at most 2^32 - 2 of them).
+struct Acc {
+  uint32_t l0 = 0, l1 = 0, l2 = 0, l3 = 0, l4 = 0;
+};
+
+// acc += a * b for any u64 a, b (the product columns as in mul).
+GL_FN void mac(Acc &acc, uint64_t a, uint64_t b) {
+  const uint32_t a0 = (uint32_t)a, a1 = (uint32_t)(a >> 32);
+  const uint32_t b0 = (uint32_t)b, b1 = (uint32_t)(b >> 32);
+  const uint64_t t0 = (uint64_t)a0 * b0;
+  const uint64_t t1 = (uint64_t)a0 * b1 + (t0 >> 32);
+  const uint64_t t2 = (uint64_t)a1 * b0 + (uint32_t)t1;
+  const uint64_t t3 = (uint64_t)a1 * b1 + (t1 >> 32);
+  const uint64_t t4 = t3 + (t2 >> 32);
+  unsigned c;
+  acc.l0 = __builtin_addc(acc.l0, (uint32_t)t0, 0u, &c);
+  acc.l1 = __builtin_addc(acc.l1, (uint32_t)t2, c, &c);
+  acc.l2 = __builtin_addc(acc.l2, (uint32_t)t4, c, &c);
+  acc.l3 = __builtin_addc(acc.l3, (uint32_t)(t4 >> 32), c, &c);
+  acc.l4 += c;
+}
+
+// Canonical value of the sum: 2^128 = 2^96 2^32 = -2^32 (mod p), and
+// l4 2^32 < p for l4 < 2^32 - 1.
+GL_FN uint64_t acc_reduce(const Acc &acc) {
+  return sub(reduce_limbs(acc.l0, acc.l1, acc.l2, acc.l3), (uint64_t)acc.l4 << 32);
+}
+
 // x * 2^E for 0 < E < 64, canonical result, any u64 x.
 template <unsigned E> GL_FN uint64_t pow2_direct(uint64_t x) {
   static_assert(E > 0 && E < 64, "");

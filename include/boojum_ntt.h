@@ -182,6 +182,50 @@ int bc_omega_shift(const uint64_t *src, uint32_t log_degree, uint32_t offset, bo
 int bc_omega_shift_in_place(uint64_t *values, uint32_t log_degree, uint32_t offset, bool inverse, uint32_t shift,
                             uint32_t count, void *stream);
 
+/* ---- Barycentric evaluation on the coset domain {coset * omega_N^i} ----
+ * Extension values are F_p[u]/(u^2 - 7) elements (c0, c1).  A single
+ * extension element (x, the common factor, one evaluation) is the pair
+ * (c0, c1) in adjacent u64; an extension vector or matrix column of N values
+ * is a "double vector": N c0 values, then N c1 values. */
+
+/* barycentric::precompute_lagrange_coeffs (src/barycentric.rs:83-127), for
+ * PrecomputeAtBase (ext false) and PrecomputeAtExt (ext true; kernels
+ * native/barycentric.cu:10-96).  x: device pointer to the point (1 or 2
+ * u64); common_factor: device scratch of the same size (receives
+ * coset (x^N - coset^N) / (N coset^N)); coeffs: N (base) or 2N (ext, double
+ * vector) u64 receiving L_i(x) = common / (x omega_N^-i - coset), in natural
+ * order or, with bit_reverse, at position brev(i).  count = N, a power of two
+ * up to 2^31.  x must not be a domain point (the basis is undefined there).
+ * Needs a context. */
+int bc_barycentric_precompute(bool ext, const uint64_t *x, uint64_t *common_factor, uint64_t coset,
+                              uint64_t *coeffs, uint64_t count, bool bit_reverse, void *stream);
+
+/* Kinds of barycentric::batch_eval (src/barycentric.rs:265-267). */
+enum {
+  BC_EVAL_BASE_AT_BASE = 0, /* EvalBaseAtBase: base columns, base coefficients */
+  BC_EVAL_BASE_AT_EXT = 1,  /* EvalBaseAtExt: base columns, extension coefficients */
+  BC_EVAL_EXT_AT_EXT = 2    /* EvalExtAtExt: extension columns and coefficients */
+};
+
+/* barycentric::get_batch_eval_temp_storage_sizes (src/barycentric.rs:196-214):
+ * partial_elems = elements (of the coefficient type: 1 u64 base, 2 ext) of
+ * the partial-sum scratch; final_bytes = bytes of the second scratch, which
+ * this implementation does not need (0).  stride = N, a power of two. */
+int bc_batch_eval_temp_sizes(uint32_t kind, uint64_t stride, uint32_t num_polys, size_t *partial_elems,
+                             size_t *final_bytes);
+
+/* barycentric::batch_eval (src/barycentric.rs:216-261): evals[p] =
+ * sum_i coeffs_i y_{i,p} for the num_polys columns of ys (column p at
+ * ys + p * stride for base columns, ys + 2 p stride as a double vector for
+ * extension columns; stride = N).  coeffs as written by
+ * bc_barycentric_precompute.  evals: num_polys u64 (BASE_AT_BASE) or
+ * num_polys (c0, c1) pairs.  temp_partial must hold partial_elems elements
+ * from bc_batch_eval_temp_sizes; temp_final may be NULL.  Raw (non-canonical)
+ * inputs are read mod p; outputs are canonical. */
+int bc_batch_eval(uint32_t kind, const uint64_t *ys, uint64_t stride, uint32_t num_polys, const uint64_t *coeffs,
+                  uint64_t *temp_partial, size_t temp_partial_elems, void *temp_final, size_t temp_final_bytes,
+                  uint64_t *evals, void *stream);
+
 /* Seeded inputs on the device: column c (c = 0..num_cols-1) at
  * out + c * stride gets 2^log_n u64 draws of the splitmix64 stream seeded with
  * seed * 0x9E3779B97F4A7C15 + first_col + c (raw u64, any value, read mod p).

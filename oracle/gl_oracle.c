@@ -347,3 +347,143 @@ void gl_fill_raw(uint64_t *out, uint64_t n, uint64_t seed, uint64_t column) {
   uint64_t st = seed * 0x9E3779B97F4A7C15ull + column;
   for (uint64_t i = 0; i < n; i++) out[i] = splitmix64(&st);
 }
+
+/* ---------------------------------------------------------------------- */
+/* Quadratic extension F_p[u]/(u^2 - 7) and barycentric evaluation.        */
+/*
+ * Restates the reference's native/barycentric.cu and src/barycentric.rs:
+ *   - common factor (barycentric.cu:12-21):
+ *       common = coset * (x^N - coset^N) / (N * coset^N)
+ *   - Lagrange coefficient i (barycentric.cu:44-86):
+ *       L_i = common / (x * omega_N^-i - coset), natural order i; with
+ *       bit_reverse the vector is then bit-reversed (src/barycentric.rs:117-124)
+ *   - batch_eval (barycentric.cu:104-156 partial sums, then the per-column
+ *     sum of src/barycentric.rs:249-260): eval_p = sum_i L_i * y_{i,p}.
+ * Extension elements are (c0, c1) = c0 + c1 u with u^2 = 7
+ * (native/goldilocks_extension.cuh, non-residue 7).  Layouts are the
+ * reference's: x and the common factor are one element (c0, c1 adjacent);
+ * coefficient vectors and extension matrices are "double vectors" (the c0
+ * plane of a column, then its c1 plane); extension evals are (c0, c1) pairs.
+ * Pinned by tests/test_oracle.py: Ext2 ops against Python big-int
+ * polynomial arithmetic, sum_i L_i(x) = 1, and batch_eval of coset
+ * evaluations against Horner's rule on the monomial coefficients.
+ */
+
+void gl_e2_mul(const uint64_t *a, const uint64_t *b, uint64_t *out) {
+  const uint64_t p00 = gl_mul(a[0], b[0]), p11 = gl_mul(a[1], b[1]);
+  const uint64_t c0 = gl_add(p00, gl_mul(7, p11));
+  const uint64_t c1 = gl_add(gl_mul(a[0], b[1]), gl_mul(a[1], b[0]));
+  out[0] = c0;
+  out[1] = c1;
+}
+
+void gl_e2_inv(const uint64_t *a, uint64_t *out) {
+  const uint64_t norm = gl_sub(gl_mul(a[0], a[0]), gl_mul(7, gl_mul(a[1], a[1])));
+  const uint64_t ni = gl_inv(norm);
+  out[0] = gl_mul(a[0], ni);
+  out[1] = gl_sub(0, gl_mul(a[1], ni));
+}
+
+/* Canonical copy of an element given as any u64 (reference fields accept
+ * raw representatives). */
+static uint64_t gl_canon(uint64_t a) { return gl_red(a); }
+
+void gl_bary_common_factor(const uint64_t *x, int ext, uint64_t coset, uint64_t count, uint64_t *out) {
+  coset = gl_canon(coset);
+  const uint64_t coset_n = gl_pow(coset, count);
+  const uint64_t denom_inv = gl_inv(gl_mul(gl_red(count), coset_n));
+  const uint64_t f = gl_mul(coset, denom_inv);
+  if (!ext) {
+    out[0] = gl_mul(gl_sub(gl_pow(gl_canon(x[0]), count), coset_n), f);
+    return;
+  }
+  uint64_t xn[2] = {1, 0}, b[2] = {gl_canon(x[0]), gl_canon(x[1])};
+  for (uint64_t e = count; e; e >>= 1) { /* square and multiply */
+    if (e & 1) gl_e2_mul(xn, b, xn);
+    gl_e2_mul(b, b, b);
+  }
+  out[0] = gl_mul(gl_sub(xn[0], coset_n), f);
+  out[1] = gl_mul(xn[1], f);
+}
+
+/* coeffs_c1 is ignored (may be NULL) when !ext. */
+void gl_bary_lagrange_coeffs(const uint64_t *x, int ext, uint64_t coset, uint32_t log_count, int bit_reverse,
+                             uint64_t *coeffs_c0, uint64_t *coeffs_c1) {
+  const uint64_t count = 1ull << log_count;
+  uint64_t common[2] = {0, 0};
+  gl_bary_common_factor(x, ext, coset, count, common);
+  coset = gl_canon(coset);
+  const uint64_t w_inv = gl_inv(gl_domain_generator(log_count));
+  const uint64_t x0 = gl_canon(x[0]), x1 = ext ? gl_canon(x[1]) : 0;
+  /* denominators d_i = x * w^-i - coset, inverted with one field inversion
+   * (prefix products, Montgomery's trick) */
+  uint64_t *d0 = malloc(count * sizeof(uint64_t)), *d1 = malloc(count * sizeof(uint64_t));
+  uint64_t *pre0 = malloc(count * sizeof(uint64_t)), *pre1 = malloc(count * sizeof(uint64_t));
+  uint64_t w = 1, acc[2] = {1, 0};
+  for (uint64_t i = 0; i < count; i++) {
+    d0[i] = gl_sub(gl_mul(x0, w), coset);
+    d1[i] = gl_mul(x1, w);
+    pre0[i] = acc[0];
+    pre1[i] = acc[1];
+    const uint64_t di[2] = {d0[i], d1[i]};
+    gl_e2_mul(acc, di, acc);
+    w = gl_mul(w, w_inv);
+  }
+  uint64_t inv_acc[2];
+  gl_e2_inv(acc, inv_acc);
+  for (uint64_t i = count; i-- > 0;) {
+    const uint64_t pi[2] = {pre0[i], pre1[i]}, di[2] = {d0[i], d1[i]};
+    uint64_t inv_i[2], l[2];
+    gl_e2_mul(inv_acc, pi, inv_i); /* 1 / d_i */
+    gl_e2_mul(inv_acc, di, inv_acc); /* 1 / (d_0 ... d_{i-1}) */
+    gl_e2_mul(inv_i, common, l);
+    const uint64_t dst = bit_reverse ? gl_bitrev(i, log_count) : i;
+    coeffs_c0[dst] = l[0];
+    if (ext) coeffs_c1[dst] = l[1];
+  }
+  free(d0);
+  free(d1);
+  free(pre0);
+  free(pre1);
+}
+
+/* evals: num_polys values (base) or (c0, c1) pairs when ys_ext or coef_ext.
+ * Base ys: column p at ys + p * count.  Extension ys: column p's c0 plane at
+ * ys + 2 p count, its c1 plane right after.  Extension coefficients: c0 plane
+ * then c1 plane (count each). */
+void gl_bary_batch_eval(const uint64_t *ys, int ys_ext, const uint64_t *coeffs, int coef_ext, uint64_t count,
+                        uint32_t num_polys, uint64_t *evals) {
+  const int out_ext = ys_ext || coef_ext;
+  for (uint32_t p = 0; p < num_polys; p++) {
+    const uint64_t *y0 = ys + (uint64_t)p * count * (ys_ext ? 2 : 1);
+    const uint64_t *y1 = ys_ext ? y0 + count : NULL;
+    uint64_t acc[2] = {0, 0};
+    for (uint64_t i = 0; i < count; i++) {
+      const uint64_t l[2] = {coeffs[i], coef_ext ? coeffs[count + i] : 0};
+      const uint64_t y[2] = {gl_canon(y0[i]), y1 ? gl_canon(y1[i]) : 0};
+      uint64_t t[2];
+      gl_e2_mul(l, y, t);
+      acc[0] = gl_add(acc[0], t[0]);
+      acc[1] = gl_add(acc[1], t[1]);
+    }
+    if (out_ext) {
+      evals[2 * p] = acc[0];
+      evals[2 * p + 1] = acc[1];
+    } else {
+      evals[p] = acc[0];
+    }
+  }
+}
+
+/* Horner's rule in the extension: sum_i (a0[i] + a1[i] u) x^i (a1 may be
+ * NULL for base coefficients). */
+void gl_e2_horner(const uint64_t *a0, const uint64_t *a1, uint64_t n, const uint64_t *x, uint64_t *out) {
+  uint64_t acc[2] = {0, 0};
+  for (uint64_t i = n; i-- > 0;) {
+    gl_e2_mul(acc, x, acc);
+    acc[0] = gl_add(acc[0], gl_canon(a0[i]));
+    acc[1] = gl_add(acc[1], a1 ? gl_canon(a1[i]) : 0);
+  }
+  out[0] = acc[0];
+  out[1] = acc[1];
+}

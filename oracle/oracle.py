@@ -54,6 +54,12 @@ def lib():
             "gl_horner": (u64, [_u64p, u64, u64]),
             "gl_fill_uniform": (None, [_u64p, u64, u64, u64]),
             "gl_fill_raw": (None, [_u64p, u64, u64, u64]),
+            "gl_e2_mul": (None, [_u64p, _u64p, _u64p]),
+            "gl_e2_inv": (None, [_u64p, _u64p]),
+            "gl_bary_common_factor": (None, [_u64p, i32, u64, u64, _u64p]),
+            "gl_bary_lagrange_coeffs": (None, [_u64p, i32, u64, u32, i32, _u64p, _u64p]),
+            "gl_bary_batch_eval": (None, [_u64p, i32, _u64p, i32, u64, u32, _u64p]),
+            "gl_e2_horner": (None, [_u64p, _u64p, u64, _u64p, _u64p]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -185,3 +191,62 @@ def canon(a):
     """Reduce a uint64 array to canonical representatives in [0, p)."""
     a = np.asarray(a, dtype=np.uint64)
     return np.where(a >= np.uint64(P), a - np.uint64(P), a)
+
+
+# ---- quadratic extension and barycentric evaluation -------------------------
+
+def _pair(x):
+    if isinstance(x, (tuple, list)):
+        return np.array([int(x[0]), int(x[1]) if len(x) > 1 else 0], dtype=np.uint64)
+    return np.array([int(x), 0], dtype=np.uint64)
+
+
+def e2_mul(a, b):
+    out = np.zeros(2, dtype=np.uint64)
+    lib().gl_e2_mul(_ptr(_pair(a)), _ptr(_pair(b)), _ptr(out))
+    return int(out[0]), int(out[1])
+
+
+def e2_inv(a):
+    out = np.zeros(2, dtype=np.uint64)
+    lib().gl_e2_inv(_ptr(_pair(a)), _ptr(out))
+    return int(out[0]), int(out[1])
+
+
+def bary_common_factor(x, coset, count):
+    """x: int (base) or (c0, c1) (extension).  Returns int or (c0, c1)."""
+    ext = isinstance(x, (tuple, list))
+    out = np.zeros(2, dtype=np.uint64)
+    lib().gl_bary_common_factor(_ptr(_pair(x)), int(ext), coset, count, _ptr(out))
+    return (int(out[0]), int(out[1])) if ext else int(out[0])
+
+
+def bary_lagrange_coeffs(x, coset, log_count, bit_reverse):
+    """Base x: a uint64 vector of 2^log_count coefficients.  Extension x
+    ((c0, c1)): the double-vector layout, c0 plane then c1 plane."""
+    ext = isinstance(x, (tuple, list))
+    n = 1 << log_count
+    out = np.zeros(2 * n if ext else n, dtype=np.uint64)
+    c1 = out[n:].ctypes.data_as(_u64p) if ext else None
+    lib().gl_bary_lagrange_coeffs(_ptr(_pair(x)), int(ext), coset, log_count, int(bit_reverse), _ptr(out), c1)
+    return out
+
+
+def bary_batch_eval(ys, ys_ext, coeffs, coef_ext, count, num_polys):
+    """Evaluations: num_polys values (base) or 2 * num_polys (c0, c1) pairs."""
+    ys = np.ascontiguousarray(ys, dtype=np.uint64)
+    coeffs = np.ascontiguousarray(coeffs, dtype=np.uint64)
+    assert ys.size >= num_polys * count * (2 if ys_ext else 1)
+    assert coeffs.size >= count * (2 if coef_ext else 1)
+    out = np.zeros(num_polys * (2 if (ys_ext or coef_ext) else 1), dtype=np.uint64)
+    lib().gl_bary_batch_eval(_ptr(ys), int(ys_ext), _ptr(coeffs), int(coef_ext), count, num_polys, _ptr(out))
+    return out
+
+
+def e2_horner(a0, a1, x):
+    """sum_i (a0[i] + a1[i] u) x^i; a1 None for base coefficients; x int or pair."""
+    a0 = np.ascontiguousarray(a0, dtype=np.uint64)
+    out = np.zeros(2, dtype=np.uint64)
+    a1p = _ptr(np.ascontiguousarray(a1, dtype=np.uint64)) if a1 is not None else None
+    lib().gl_e2_horner(_ptr(a0), a1p, a0.size, _ptr(_pair(x)), _ptr(out))
+    return int(out[0]), int(out[1])

@@ -1,9 +1,11 @@
 // CPU check of the device field arithmetic (era-boojum-cuda_amd/csrc/gl_field.hpp,
 // compiled here as host code) against 128-bit integer arithmetic mod
 // p = 2^64 - 2^32 + 1: canon, add, sub, neg, mul, mul by 2^e for all 192
-// exponents (2 has order 192), pow and inv, on edge values and random draws
+// exponents (2 has order 192), pow and inv, the unreduced multiply-accumulate
+// (mac / acc_reduce) over sums of up to 4096 products, and the quadratic
+// extension F_p[u]/(u^2 - 7) of gl_ext.hpp, on edge values and random draws
 // biased toward carries.  Prints "OK" and exits 0 when every result matches.
-#include "../../era-boojum-cuda_amd/csrc/gl_field.hpp"
+#include "../../era-boojum-cuda_amd/csrc/gl_ext.hpp"
 
 #include <stdio.h>
 
@@ -73,6 +75,32 @@ int main() {
     const uint64_t a = draw(g, i) % P, e = g();
     if (gl::pow(a, e) != ref_pow(a, e)) report("pow", a, e, gl::pow(a, e), ref_pow(a, e));
     if (a && ref_mul(gl::inv(a), a) != 1) report("inv", a, 0, gl::inv(a), 0);
+  }
+  // mac: sums of n products of raw u64 values, n up to 4096 (carry counts
+  // up to 4096 in the top limb), against the sum of reduced products.
+  for (int trial = 0; trial < 3000; ++trial) {
+    const int n = trial < 2000 ? 1 + trial % 40 : 4096;
+    gl::Acc acc;
+    uint64_t want = 0;
+    for (int i = 0; i < n; ++i) {
+      const uint64_t a = trial % 3 ? draw(g, trial + i) : ~0ull, b = trial % 5 ? draw(g, i * 7 + trial) : ~0ull;
+      gl::mac(acc, a, b);
+      want = (uint64_t)(((u128)want + ref_mul(a, b)) % P);
+    }
+    if (gl::acc_reduce(acc) != want) report("mac", (uint64_t)n, 0, gl::acc_reduce(acc), want);
+  }
+  // quadratic extension: product by schoolbook arithmetic, inverse by a * a^-1 = 1
+  for (int i = 0; i < 200000; ++i) {
+    const gl::E2 a{draw(g, i) % P, draw(g, i + 3) % P}, b{draw(g, i + 11) % P, draw(g, i / 2) % P};
+    const gl::E2 m = gl::e2_mul(a, b);
+    const uint64_t w0 = (uint64_t)(((u128)ref_mul(a.c0, b.c0) + ref_mul(7, ref_mul(a.c1, b.c1))) % P);
+    const uint64_t w1 = (uint64_t)(((u128)ref_mul(a.c0, b.c1) + ref_mul(a.c1, b.c0)) % P);
+    if (m.c0 != w0 || m.c1 != w1) report("e2_mul", a.c0, b.c0, m.c0, w0);
+    if (gl::mul_by_non_residue(a.c1) != ref_mul(7, a.c1)) report("nr", a.c1, 0, gl::mul_by_non_residue(a.c1), 0);
+    if (i < 2000 && (a.c0 | a.c1)) {
+      const gl::E2 r = gl::e2_mul(gl::e2_inv(a), a);
+      if (r.c0 != 1 || r.c1 != 0) report("e2_inv", a.c0, a.c1, r.c0, 1);
+    }
   }
   if (bad) {
     printf("FAIL %d mismatches\n", bad);
