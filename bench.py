@@ -237,6 +237,56 @@ def bench_ops(log_n, cols, reps):
     return {"workload": f"{cols} x 2^{log_n} u64 ({total} elements)", **out}
 
 
+def bench_barycentric(log_n, cols, reps):
+    """Barycentric evaluation on the batched shape: Lagrange coefficients at a
+    base / extension point (8 / 16 B written per element) and batch_eval of
+    `cols` base columns (BaseAtBase, BaseAtExt) or cols / 2 extension columns
+    (ExtAtExt) of 2^log_n rows: algorithmic bytes = the columns read once plus
+    the coefficients (8 or 16 B per row).  GB/s against the 8 TB/s peak."""
+    import torch
+    from boojum_cuda_amd import barycentric as bary
+    n = 1 << log_n
+    g = torch.Generator(device="cuda")
+    g.manual_seed(8)
+    ys = uniform_field(cols * n, g)
+    stream = torch.cuda.current_stream()
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2 * reps)]
+        for r in range(reps):
+            ev[2 * r].record(stream)
+            fn()
+            ev[2 * r + 1].record(stream)
+        torch.cuda.synchronize()
+        return sum(ev[2 * r].elapsed_time(ev[2 * r + 1]) for r in range(reps)) / reps
+
+    out = {}
+    x = uniform_field(2, g)
+    common = torch.empty(2, dtype=torch.int64, device="cuda")
+    coeffs = {False: torch.empty(n, dtype=torch.int64, device="cuda"),
+              True: torch.empty(2 * n, dtype=torch.int64, device="cuda")}
+    for ext in (False, True):
+        ms = timed(lambda: bary.precompute_lagrange_coeffs(ext, x, common, 7, coeffs[ext], True))
+        gbs = (16 if ext else 8) * n / (ms * 1e-3) / 1e9
+        out["precompute_at_ext" if ext else "precompute_at_base"] = {"ms": ms, "GB/s": gbs, "frac": gbs / HBM_PEAK_GBS}
+    for name, kind, polys in [("base_at_base", bary.EVAL_BASE_AT_BASE, cols),
+                              ("base_at_ext", bary.EVAL_BASE_AT_EXT, cols),
+                              ("ext_at_ext", bary.EVAL_EXT_AT_EXT, cols // 2)]:
+        w = 1 if kind == bary.EVAL_BASE_AT_BASE else 2
+        partial, _ = bary.get_batch_eval_temp_storage_sizes(kind, n, polys)
+        temp = torch.empty(partial * w, dtype=torch.int64, device="cuda")
+        evals = torch.empty(polys * w, dtype=torch.int64, device="cuda")
+        L = coeffs[kind != bary.EVAL_BASE_AT_BASE]
+        ms = timed(lambda: bary.batch_eval(kind, ys, n, polys, L, temp, None, evals))
+        algo = 8 * cols * n + 8 * w * n
+        gbs = algo / (ms * 1e-3) / 1e9
+        out[name] = {"ms": ms, "GB/s": gbs, "frac": gbs / HBM_PEAK_GBS, "polys": polys}
+    del ys
+    return {"workload": f"{cols} base columns (or {cols // 2} extension columns) x 2^{log_n}", **out}
+
+
 def cpu_baseline(log_n, cols, threads):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import numpy as np
@@ -282,6 +332,7 @@ def main():
     ap.add_argument("--ragged-jobs", type=int, default=1000)
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop measurement")
     ap.add_argument("--no-ops", action="store_true", help="skip the bit-reverse / powers / omega-shift measurement")
+    ap.add_argument("--no-bary", action="store_true", help="skip the barycentric-evaluation measurement")
     ap.add_argument("--serve-requests", type=int, default=100)
     ap.add_argument("--serve-rate", type=float, default=25.0, help="offered load, requests per second")
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
@@ -362,6 +413,10 @@ def main():
         torch.cuda.empty_cache()
         extras["ops"] = bench_ops(log_n, cols, 5)
         progress(f"ops done: bit_reverse {extras['ops']['bit_reverse']['GB/s']:.0f} GB/s")
+    if not args.no_bary:
+        torch.cuda.empty_cache()
+        extras["barycentric"] = bench_barycentric(log_n, cols, 5)
+        progress(f"barycentric done: base_at_ext {extras['barycentric']['base_at_ext']['GB/s']:.0f} GB/s")
     if not args.no_serve:
         torch.cuda.empty_cache()
         extras["serve"] = bench_serve(args.serve_requests, args.serve_rate, int(args.serve_budget_gb * 1e9))

@@ -71,10 +71,12 @@ int main() {
     if (gl::neg(ca) != (P - ca) % P) report("neg", ca, 0, gl::neg(ca), (P - ca) % P);
     if (gl::mul(a, b) != ref_mul(a, b)) report("mul", a, b, gl::mul(a, b), ref_mul(a, b));
   }
+  if (gl::inv(0) != 0 || gl::inv(P) != 0 || gl::inv(P + 1) != 1) report("inv edge", 0, 0, gl::inv(0), 0);
   for (int i = 0; i < 2000; ++i) {
     const uint64_t a = draw(g, i) % P, e = g();
     if (gl::pow(a, e) != ref_pow(a, e)) report("pow", a, e, gl::pow(a, e), ref_pow(a, e));
     if (a && ref_mul(gl::inv(a), a) != 1) report("inv", a, 0, gl::inv(a), 0);
+    if (gl::inv(a) != ref_pow(a, P - 2)) report("inv chain", a, 0, gl::inv(a), ref_pow(a, P - 2));
   }
   // mac: sums of n products of raw u64 values, n up to 4096 (carry counts
   // up to 4096 in the top limb), against the sum of reduced products.

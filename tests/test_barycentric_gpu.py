@@ -95,7 +95,7 @@ def eval_gpu(kind, ys, log_count, num_polys, coeffs, misalign=False):
     n = 1 << log_count
     out_w = 1 if kind == 0 else 2
     partial, final = bary.get_batch_eval_temp_storage_sizes(kind, n, num_polys)
-    assert final == 0 and partial == num_polys * max(1, n // 128)
+    assert final == 0 and partial >= num_polys
     temp = torch.full((partial * out_w,), -1, dtype=torch.int64, device="cuda")
     evals = torch.zeros(num_polys * out_w, dtype=torch.int64, device="cuda")
     if misalign:  # 8-byte aligned views: the two-load path
@@ -139,7 +139,7 @@ def test_batch_eval_matches_horner(gpu_ctx, kind, num_polys):
 
 
 @pytest.mark.parametrize("kind", list(KINDS))
-@pytest.mark.parametrize("log_count", [6, 7, 10])
+@pytest.mark.parametrize("log_count", [6, 7, 8, 10, 17, 18])
 def test_batch_eval_raw_misaligned(gpu_ctx, kind, log_count):
     """Raw u64 column values (>= p allowed) and 8-byte-aligned buffers, against
     the oracle's sum over the same inputs."""
