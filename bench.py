@@ -399,6 +399,16 @@ def main():
     value = elems_per_step / (ms_per_step * 1e-3)
     algo_bytes = 16.0 * cols * n
     achieved = algo_bytes / (avg_call_ms * 1e-3) / 1e9
+    # measured copy peak: a device-to-device copy of the same 2^28-element
+    # matrix (16 B moved per element), timed like the transform calls
+    y.copy_(x)
+    cev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    cev[0].record(stream)
+    for _ in range(5):
+        y.copy_(x)
+    cev[1].record(stream)
+    torch.cuda.synchronize()
+    copy_gbs = algo_bytes / (cev[0].elapsed_time(cev[1]) / 5 * 1e-3) / 1e9
     del x, y
     progress(f"batched NTT done: {avg_call_ms:.3f} ms per call")
     extras = {}
@@ -442,7 +452,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(log_n, cols),
                      "kernel": "one batch_ntt call (all its pass kernels)", "avg_call_ms": avg_call_ms,
-                     "algorithmic_bytes_per_call": algo_bytes},
+                     "algorithmic_bytes_per_call": algo_bytes, "measured_copy_peak": copy_gbs,
+                     "frac_of_copy_peak": achieved / copy_gbs},
     }
     rec.update(extras)
     if not args.no_cpu:

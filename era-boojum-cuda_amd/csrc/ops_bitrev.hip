@@ -98,7 +98,10 @@ int bitrev_launch(const uint64_t *src, uint64_t stride_src, uint64_t *dst, uint6
   const uint32_t log_n = (uint32_t)__builtin_ctzll(rows);
   if (log_n >= 10) {
     const uint64_t work = ((uint64_t)cols) << (log_n - 10);
-    const uint32_t grid = (uint32_t)(work < 65536 ? work : 65536);
+    // An odd grid stride walks each workgroup through all middle indices m,
+    // so in place the skipped half (m > brev(m)) is spread evenly instead of
+    // idling the workgroups whose m is fixed by a power-of-two stride.
+    const uint32_t grid = (uint32_t)(work < 65535 ? work : 65535);
     if (in_place) bitrev_tiled_kernel<true><<<grid, BR_NT, 0, s>>>(dst, dst, stride_dst, stride_dst, log_n, cols);
     else bitrev_tiled_kernel<false><<<grid, BR_NT, 0, s>>>(src, dst, stride_src, stride_dst, log_n, cols);
   } else {
