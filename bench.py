@@ -287,6 +287,56 @@ def bench_barycentric(log_n, cols, reps):
     return {"workload": f"{cols} base columns (or {cols // 2} extension columns) x 2^{log_n}", **out}
 
 
+def bench_latency(reps=200):
+    """Config 1: one column of 2^16, forward natural -> bit-reversed, out of
+    place.  Reports the GPU time per call (HIP events, back to back), the
+    host-observed latency of a call plus a stream synchronize (median), and
+    whether the inverse round trip returns the input."""
+    import torch
+    from boojum_cuda_amd import ntt
+    log_n = 16
+    n = 1 << log_n
+    g = torch.Generator(device="cuda")
+    g.manual_seed(1)
+    x = uniform_field(n, g)
+    y = torch.empty_like(x)
+    z = torch.empty_like(x)
+    stream = torch.cuda.current_stream()
+    for _ in range(10):
+        ntt.batch_ntt_out_of_place(x, y, log_n, 1, 0, 0, n, n, False, False, 0, 0)
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    ev[0].record(stream)
+    for _ in range(reps):
+        ntt.batch_ntt_out_of_place(x, y, log_n, 1, 0, 0, n, n, False, False, 0, 0)
+    ev[1].record(stream)
+    torch.cuda.synchronize()
+    gpu_us = ev[0].elapsed_time(ev[1]) * 1e3 / reps
+    lat = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        ntt.batch_ntt_out_of_place(x, y, log_n, 1, 0, 0, n, n, False, False, 0, 0)
+        stream.synchronize()
+        lat.append((time.perf_counter() - t0) * 1e6)
+    lat.sort()
+    ntt.batch_ntt_out_of_place(y, z, log_n, 1, 0, 0, n, n, True, True, 0, 0)  # b2n inverse
+    torch.cuda.synchronize()
+    return {"workload": "1 x 2^16 forward n2b, out of place", "gpu_us_per_call": gpu_us,
+            "host_latency_us_median": lat[len(lat) // 2], "host_latency_us_p99": lat[int(0.99 * (len(lat) - 1))],
+            "inverse_round_trip_exact": bool(torch.equal(z, x))}
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(log_n, cols, threads):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import numpy as np
@@ -297,8 +347,19 @@ def cpu_baseline(log_n, cols, threads):
     t0 = time.perf_counter()
     oracle.batch_ntt(inp, log_n, cols, n, n, False, False, out=out, threads=threads)
     dt = time.perf_counter() - t0
+    # config 1 on one thread: 1 x 2^16 forward n2b, 3 warm-up + 15 timed runs, median
+    one = oracle.matrix_uniform(1, 1 << 16, 1)
+    runs = []
+    for k in range(18):
+        t1 = time.perf_counter()
+        oracle.batch_ntt(one, 16, 1, 1 << 16, 1 << 16, False, False)
+        if k >= 3:
+            runs.append(time.perf_counter() - t1)
+    runs.sort()
     return {"value": cols * n / dt, "unit": "elem/s", "cores": threads, "kind": "port",
-            "sample": f"{cols} columns x 2^{log_n}, forward n2b, {dt:.2f} s wall"}
+            "sample": f"{cols} columns x 2^{log_n}, forward n2b, {dt:.2f} s wall",
+            "nproc": os.cpu_count(), "cpu_model": cpu_model(),
+            "config1_single_thread_ms_median": runs[len(runs) // 2] * 1e3}
 
 
 def measured_traffic(log_n, cols):
@@ -326,7 +387,7 @@ def main():
     ap.add_argument("--log-n", type=int, default=20)
     ap.add_argument("--cols", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-cols", type=int, default=64)
+    ap.add_argument("--cpu-cols", type=int, default=128)
     ap.add_argument("--no-lde", action="store_true", help="skip the coset-LDE measurement")
     ap.add_argument("--no-ragged", action="store_true", help="skip the ragged-batch measurement")
     ap.add_argument("--ragged-jobs", type=int, default=1000)
@@ -412,6 +473,8 @@ def main():
     del x, y
     progress(f"batched NTT done: {avg_call_ms:.3f} ms per call")
     extras = {}
+    extras["latency"] = bench_latency()
+    progress(f"latency done: {extras['latency']['gpu_us_per_call']:.1f} us per 2^16 call")
     if not args.no_lde:
         extras["lde"] = bench_lde(20, 128, 2, max(3, args.steps // 2), 2)
         torch.cuda.empty_cache()
