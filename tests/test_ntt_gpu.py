@@ -114,6 +114,15 @@ def test_correctness_batch_lde_large(gpu_ctx, log_n, inverse):
     correctness([log_n], inverse, 3, 3, 3)
 
 
+@pytest.mark.parametrize("log_n,inverse,log_ext,coset", [(24, False, 0, 0), (24, True, 0, 0), (25, False, 0, 0),
+                                                          (26, True, 0, 0), (24, False, 2, 3), (23, True, 3, 5)])
+def test_correctness_max_sizes(gpu_ctx, log_n, inverse, log_ext, coset):
+    """log_n 24 is the reference's limit (OMEGA_LOG_ORDER, src/context.rs:12,
+    src/ntt.rs:193); single columns and coset extensions up to 2^26 points are
+    the sizes the ragged and serving configurations need."""
+    correctness([log_n], inverse, log_ext, coset, 1)
+
+
 @pytest.mark.parametrize("k,c", [(1, 0), (1, 1), (2, 0), (2, 3), (4, 9)])
 def test_lde_cosets(gpu_ctx, k, c):
     correctness([5, 12, 14], False, k, c, 2)
