@@ -32,12 +32,13 @@ GL_FN uint64_t canon(uint64_t x) {
   return t < x ? t : x;
 }
 
-// a + b mod p for CANONICAL a, b; result canonical.  a + b < 2p, so either the
-// 64-bit add wraps (result a + b - 2^64 + EPS, no second wrap) or the sum is
-// below 2^64 and at most one p comes off (detected by s + EPS wrapping).
+// a + b mod p for CANONICAL a, b; result canonical.  b + EPS = b - p + 2^64
+// does not wrap (b <= p - 1 = 2^64 - EPS - 1), so r = a + (b + EPS) wraps
+// (r < a) exactly when a + b >= p, and r is then a + b - p: one 64-bit
+// compare decides, with no second wrap test.
 GL_FN uint64_t add(uint64_t a, uint64_t b) {
-  const uint64_t s = a + b, t = s + EPS;
-  return ((s < a) | (t < s)) ? t : s;
+  const uint64_t s = a + b, r = a + (b + EPS);
+  return r < a ? r : s;
 }
 
 // a - b mod p for CANONICAL a, b; result canonical.  The borrow comes from
