@@ -394,6 +394,7 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop measurement")
     ap.add_argument("--no-ops", action="store_true", help="skip the bit-reverse / powers / omega-shift measurement")
     ap.add_argument("--no-bary", action="store_true", help="skip the barycentric-evaluation measurement")
+    ap.add_argument("--no-latency", action="store_true", help="skip the single-column 2^16 latency measurement")
     ap.add_argument("--serve-requests", type=int, default=100)
     ap.add_argument("--serve-rate", type=float, default=25.0, help="offered load, requests per second")
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
@@ -473,8 +474,9 @@ def main():
     del x, y
     progress(f"batched NTT done: {avg_call_ms:.3f} ms per call")
     extras = {}
-    extras["latency"] = bench_latency()
-    progress(f"latency done: {extras['latency']['gpu_us_per_call']:.1f} us per 2^16 call")
+    if not args.no_latency:
+        extras["latency"] = bench_latency()
+        progress(f"latency done: {extras['latency']['gpu_us_per_call']:.1f} us per 2^16 call")
     if not args.no_lde:
         extras["lde"] = bench_lde(20, 128, 2, max(3, args.steps // 2), 2)
         torch.cuda.empty_cache()
