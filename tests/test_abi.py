@@ -102,3 +102,18 @@ def test_python_mirror_raises_assertion_on_bad_args():
     from boojum_cuda_amd import ntt
     with pytest.raises(AssertionError):
         ntt.batch_ntt_internal(FAKE_IN, FAKE_OUT, 0, 1, 16, 16, False, False, 0, 0, None)
+
+
+def test_missing_library_fails_loudly():
+    """No fallback: with the HIP library absent, the first call into the
+    package raises instead of computing anything on the CPU."""
+    import subprocess
+    import sys
+    code = ("import boojum_cuda_amd._lib as L; L.LIB_PATH = '/nonexistent/libboojum_ntt.so'\n"
+            "from boojum_cuda_amd import ntt\n"
+            "try:\n    ntt.batch_ntt_internal(0, 0, 4, 1, 16, 16, False, False, 0, 0, 0)\n"
+            "except ImportError as e:\n    print('raised', e)\n")
+    pkg = os.path.join(ROOT, "era-boojum-cuda_amd")
+    out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120,
+                         env={**os.environ, "PYTHONPATH": pkg})
+    assert out.returncode == 0 and out.stdout.startswith("raised"), out.stdout + out.stderr
