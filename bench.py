@@ -387,7 +387,7 @@ def main():
     ap.add_argument("--log-n", type=int, default=20)
     ap.add_argument("--cols", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-cols", type=int, default=128)
+    ap.add_argument("--cpu-cols", type=int, default=256)
     ap.add_argument("--no-lde", action="store_true", help="skip the coset-LDE measurement")
     ap.add_argument("--no-ragged", action="store_true", help="skip the ragged-batch measurement")
     ap.add_argument("--ragged-jobs", type=int, default=1000)
