@@ -179,8 +179,9 @@ def tune_shape(log_n, log_cols, reps=5, log=print):
             ms = trial(p)
             if better(ms, best_ms):
                 best, best_ms = p, ms
-    # 3. contiguous grid cap
-    for grid in (512, 1024, 4096, 8192):
+    # 3. contiguous grid cap (the built-in default is two rounds of resident
+    # workgroups; 768 and 2304 are one and three rounds on 256 CUs x 3)
+    for grid in (512, 768, 1024, 2304, 4096, 8192):
         p = TuneParams(*best.key())
         p.contig_grid = grid
         ms = trial(p)
