@@ -26,6 +26,8 @@
 #include "gl_ext.hpp"
 #include "ntt_device.hpp"
 
+#include <mutex>
+
 namespace bc {
 
 Tables current_tables(bool &ok);  // ntt_api.hip
@@ -418,6 +420,26 @@ uint64_t eval_tiles(uint32_t kind, uint64_t count) {
   return rows ? count / rows : 0;
 }
 
+// Workgroups of one tile-kernel variant resident on the current device (CUs x
+// workgroups per CU at its register use), queried once per device.
+template <uint32_t KIND, class S> uint32_t resident_wgs() {
+  constexpr int MAX_DEVICES = 64;
+  static std::mutex mu;
+  static uint32_t cache[MAX_DEVICES] = {};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= MAX_DEVICES) return 768;
+  std::lock_guard<std::mutex> lk(mu);
+  if (!cache[dev]) {
+    int cus = 0, per_cu = 0;
+    const bool ok = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+                    hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, bary_eval_tiles_kernel<KIND, true, S>,
+                                                                 BE_NT, 0) == hipSuccess &&
+                    cus > 0 && per_cu > 0;
+    cache[dev] = ok ? (uint32_t)(cus * per_cu) : 768u;
+  }
+  return cache[dev];
+}
+
 // Launch grid: tiles x column-pass groups, about `target` workgroups (more
 // for the short-column shapes, whose tiles are few).
 template <uint32_t KIND, class S>
@@ -447,8 +469,9 @@ void launch_eval(const uint64_t *ys, const uint64_t *coeffs, uint64_t count, uin
   if (rows == 0)
     bary_eval_small_kernel<KIND><<<dim3((num_polys + 255) / 256), 256, 0, s>>>(ys, coeffs, (uint32_t)count,
                                                                                num_polys, evals);
-  else if (rows == T::large::rows && count >= BE_LARGE_COUNT)
-    launch_tiles<KIND, typename T::large>(ys, coeffs, count, num_polys, partial, evals, 2048, s);
+  else if (rows == T::large::rows && count >= BE_LARGE_COUNT)  // about three whole rounds of resident workgroups
+    launch_tiles<KIND, typename T::large>(ys, coeffs, count, num_polys, partial, evals,
+                                          3ull * resident_wgs<KIND, typename T::large>(), s);
   else if (rows == T::mid::rows)
     launch_tiles<KIND, typename T::mid>(ys, coeffs, count, num_polys, partial, evals, 8192, s);
   else
