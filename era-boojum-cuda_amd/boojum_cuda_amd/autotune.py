@@ -117,6 +117,14 @@ class _Bench:
         self.torch.cuda.synchronize()
         return self.y.clone(), self.z.clone()
 
+    def warm(self, seconds=0.15):
+        """Runs the shape until `seconds` have passed, so the first timed
+        configuration does not pay for clock ramp-up."""
+        t0 = time.perf_counter()
+        while time.perf_counter() - t0 < seconds:
+            self.run()
+            self.torch.cuda.synchronize()
+
 
 def _time_ragged(log_n, num_cols, reps):
     """One ragged launch of an NTT job and an iNTT job of this shape."""
@@ -154,6 +162,7 @@ def tune_shape(log_n, log_cols, reps=5, log=print):
     set_entry(log_n, log_cols, None)
     bench = _Bench(log_n, num_cols, reps)
     ref = bench.outputs()
+    bench.warm()
     base_ms = bench.time_ms()
     best = TuneParams()
     best_ms = base_ms
@@ -187,8 +196,21 @@ def tune_shape(log_n, log_cols, reps=5, log=print):
         ms = trial(p)
         if better(ms, best_ms):
             best, best_ms = p, ms
+    # confirmation: the built-in plan and the winner timed alternately, so a
+    # gain that came from timing noise (or from a plan equal to the built-in
+    # one) is not recorded
+    if best.key() != TuneParams().key():
+        b_ms, w_ms = [], []
+        for _ in range(3):
+            set_entry(log_n, log_cols, None)
+            b_ms.append(bench.time_ms())
+            set_entry(log_n, log_cols, best)
+            w_ms.append(bench.time_ms())
+        if not better(min(w_ms), min(b_ms)):
+            best, best_ms = TuneParams(), min(b_ms)
     # 4. ragged work units (timed on the persistent path)
     set_entry(log_n, log_cols, best)
+    _time_ragged(log_n, num_cols, 1)  # warm the persistent path
     r_best = _time_ragged(log_n, num_cols, reps)
     for uc in (4, 8, 16, 32, 64):
         for us in (4, 8, 16, 32, 64):
@@ -199,6 +221,17 @@ def tune_shape(log_n, log_cols, reps=5, log=print):
             if better(ms, r_best):
                 r_best = ms
                 best.ragged_unit_contig, best.ragged_unit_strided = uc, us
+    if (best.ragged_unit_contig, best.ragged_unit_strided) != (0, 0):  # confirm as above
+        plain = TuneParams(*best.key())
+        plain.ragged_unit_contig = plain.ragged_unit_strided = 0
+        d_ms, u_ms = [], []
+        for _ in range(3):
+            set_entry(log_n, log_cols, plain)
+            d_ms.append(_time_ragged(log_n, num_cols, reps))
+            set_entry(log_n, log_cols, best)
+            u_ms.append(_time_ragged(log_n, num_cols, reps))
+        if not better(min(u_ms), min(d_ms)):
+            best = plain
     # keep the entry only if it differs from the built-in plan and gives the same bits
     if best.key() == (0, 0, 0, 0, 0, 0):
         set_entry(log_n, log_cols, None)
