@@ -14,8 +14,9 @@ launches on (torch's current stream) over the timed region.
 cpu_baseline: the oracle's multithreaded restatement of boojum's CPU NTT
 (oracle/gl_oracle.c), timed on a bounded sample of the same workload.
 
-Prints ONE JSON line.  `--gpus` above 1 is refused: this build measures one
-GPU's share (columns are independent; see DESIGN.md for sharding).
+Prints ONE JSON line.  `--gpus` above 1 is refused: the measurement is one
+GPU's share (columns are independent, so a batch shards by column range with
+no communication).
 """
 import argparse
 import json
@@ -402,8 +403,8 @@ def main():
                     help="copy: time a plain device copy of the same matrix (calibrates HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (columns shard with no "
-              f"communication; see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (columns shard by range "
+              f"with no communication, so one GPU's share is the measurement)", file=sys.stderr)
         sys.exit(2)
 
     import torch

@@ -72,7 +72,7 @@ int bc_batch_ntt_in_place(uint64_t *inputs_matrix, size_t inputs_len, uint32_t l
                           uint32_t inputs_offset, uint32_t stride_between_input_arrays, bool bitrev_inputs,
                           bool inverse, uint32_t log_extension_degree, uint32_t coset_index, void *stream);
 
-/* ---- Extensions beyond the reference API (north-star pipeline pieces) ---- */
+/* ---- Extensions beyond the reference API (the pipeline around the NTT) ---- */
 
 /* Coset low-degree extension of a batch of columns.  No single reference
  * function does this; it chains the reference's own calls the way its users
