@@ -1,8 +1,10 @@
 /*
  * C ABI of the MI355X (gfx950) batched Goldilocks NTT / coset-LDE library
  * (libboojum_ntt.so).  This is the drop-in boundary for boojum-cuda's `ntt`
- * and `context` modules: a Rust FFI crate (or any C caller) binds exactly
- * these symbols in place of the CUDA kernels that era_cudart launched.
+ * and `context` modules and for the functions either side of the transform
+ * (`ops_complex` bit reversal, domain powers and omega shift; `barycentric`
+ * evaluation): a Rust FFI crate (or any C caller) binds exactly these symbols
+ * in place of the CUDA kernels that era_cudart launched.
  *
  * Conventions (same as the reference):
  *   - Elements are Goldilocks field elements stored as uint64_t, any value
