@@ -77,6 +77,19 @@ def splits(log_n):
     return out
 
 
+def builtin_split(log_n):
+    """The split the library uses without a tuned entry (make_fast_plan in
+    csrc/ntt_api.hip): a 12-bit contiguous pass and the fewest strided passes
+    of at most 8 bits, the contiguous pass shrunk when the rest is under 4."""
+    if log_n <= 12:
+        return (log_n, 0)
+    c, s = 12, log_n - 12
+    k = (s + 7) // 8
+    if s < 4 * k:
+        c -= 4 * k - s
+    return (c, k)
+
+
 class _Bench:
     """Buffers and timing of one (log_n, num_cols) shape."""
 
@@ -174,8 +187,11 @@ def tune_shape(log_n, log_cols, reps=5, log=print):
     def better(ms, incumbent):
         return ms < incumbent * (1.0 - MIN_GAIN)
 
-    # 1. pass split
+    # 1. pass split (the built-in split itself is not a candidate: an entry
+    # that repeats it could only record timing noise)
     for c, k in splits(log_n):
+        if (c, k) == builtin_split(log_n):
+            continue
         p = TuneParams(c, k, 0, 0, 0, 0)
         ms = trial(p)
         if better(ms, best_ms):

@@ -89,3 +89,16 @@ def test_shipped_database_is_valid(tune):
     if not os.path.exists(path):
         pytest.skip("no tuning database shipped")
     tune.load(path)
+
+
+def test_builtin_split_is_a_valid_split_and_matches_the_planner_rule():
+    """The tuner skips the library's own split (make_fast_plan): 12 contiguous
+    bits and the fewest strided passes of 4..8 bits, the contiguous pass
+    shrunk when fewer than 4 bits remain."""
+    from boojum_cuda_amd import autotune
+    want = {12: (12, 0), 13: (9, 1), 15: (11, 1), 16: (12, 1), 20: (12, 1), 21: (12, 2), 24: (12, 2),
+            28: (12, 2), 29: (12, 3)}
+    for log_n, split in want.items():
+        assert autotune.builtin_split(log_n) == split, log_n
+    for log_n in range(13, 33):
+        assert autotune.builtin_split(log_n) in autotune.splits(log_n), log_n
