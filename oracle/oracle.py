@@ -1,4 +1,5 @@
-"""ctypes wrapper around the CPU oracle (oracle/gl_oracle.c).
+"""ctypes wrapper around the CPU oracle (oracle/gl_oracle.c and
+oracle/poseidon_oracle.c).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
 cpu_baseline leg of bench.py, never by the product package.  The C file's
@@ -60,6 +61,13 @@ def lib():
             "gl_bary_lagrange_coeffs": (None, [_u64p, i32, u64, u32, i32, _u64p, _u64p]),
             "gl_bary_batch_eval": (None, [_u64p, i32, _u64p, i32, u64, u32, _u64p]),
             "gl_e2_horner": (None, [_u64p, _u64p, u64, _u64p, _u64p]),
+            "pos_permute_textbook": (None, [_u64p, _u64p]),
+            "pos_fused_words": (ctypes.c_size_t, []),
+            "pos_derive": (i32, [_u64p, _u64p]),
+            "pos_permute_fused": (None, [_u64p, _u64p, _u64p]),
+            "pos_leaves": (None, [_u64p, _u64p, u32, u32, u32, i32, i32, _u64p, _u64p]),
+            "pos_nodes": (None, [_u64p, _u64p, u32, _u64p, _u64p]),
+            "pos_grain_constants": (None, [_u64p]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -250,3 +258,63 @@ def e2_horner(a0, a1, x):
     a1p = _ptr(np.ascontiguousarray(a1, dtype=np.uint64)) if a1 is not None else None
     lib().gl_e2_horner(_ptr(a0), a1p, a0.size, _ptr(_pair(x)), _ptr(out))
     return int(out[0]), int(out[1])
+
+
+# ---- Poseidon (oracle/poseidon_oracle.c) ------------------------------------
+
+POS_T, POS_RATE, POS_CAP, POS_PARTIAL = 12, 8, 4, 22
+
+
+def pos_grain_constants():
+    out = np.zeros(360, dtype=np.uint64)
+    lib().pos_grain_constants(_ptr(out))
+    return out
+
+
+def pos_derive(rc):
+    """Fused form of 360 textbook constants: dict of rcf (12), dense (12 x 12),
+    sbox_c (22), vs (22 x 11), w_hats (22 x 11), plus the packed words."""
+    rc = np.ascontiguousarray(rc, dtype=np.uint64)
+    words = np.zeros(lib().pos_fused_words(), dtype=np.uint64)
+    assert lib().pos_derive(_ptr(rc), _ptr(words)) == 1
+    o = 0
+    out = {"words": words}
+    for name, shape in (("rcf", (12,)), ("dense", (12, 12)), ("sbox_c", (22,)), ("vs", (22, 11)),
+                        ("w_hats", (22, 11))):
+        k = int(np.prod(shape))
+        out[name] = words[o:o + k].reshape(shape)
+        o += k
+    return out
+
+
+def pos_permute_textbook(state, rc):
+    s = np.ascontiguousarray(state, dtype=np.uint64).copy()
+    lib().pos_permute_textbook(_ptr(s), _ptr(np.ascontiguousarray(rc, dtype=np.uint64)))
+    return s
+
+
+def pos_permute_fused(state, rc, fused_words):
+    s = np.ascontiguousarray(state, dtype=np.uint64).copy()
+    lib().pos_permute_fused(_ptr(s), _ptr(np.ascontiguousarray(rc, dtype=np.uint64)), _ptr(fused_words))
+    return s
+
+
+def pos_leaves(values, rows, cols, count, rc, fused_words, results=None, load_intermediate=False,
+               store_intermediate=False):
+    """Leaf digests, 4 columns of `count` (results updated in place when given)."""
+    values = np.ascontiguousarray(values, dtype=np.uint64)
+    assert values.size >= rows * cols * count
+    if results is None:
+        results = np.zeros(4 * count, dtype=np.uint64)
+    lib().pos_leaves(_ptr(values), _ptr(results), rows, cols, count, int(load_intermediate),
+                     int(store_intermediate), _ptr(np.ascontiguousarray(rc, dtype=np.uint64)), _ptr(fused_words))
+    return results
+
+
+def pos_nodes(values, count, rc, fused_words):
+    values = np.ascontiguousarray(values, dtype=np.uint64)
+    assert values.size >= 8 * count
+    out = np.zeros(4 * count, dtype=np.uint64)
+    lib().pos_nodes(_ptr(values), _ptr(out), count, _ptr(np.ascontiguousarray(rc, dtype=np.uint64)),
+                    _ptr(fused_words))
+    return out
