@@ -288,6 +288,47 @@ def bench_barycentric(log_n, cols, reps):
     return {"workload": f"{cols} base columns (or {cols // 2} extension columns) x 2^{log_n}", **out}
 
 
+def bench_poseidon(log_n, cols, reps):
+    """Poseidon Merkle tree over one LDE coset: `cols` columns x 2^log_n rows,
+    one leaf per row (cols / 8 permutations per leaf), then the 2^log_n - 1
+    node hashes.  The permutation is VALU-bound; GB/s is the leaf input read
+    once.  Round constants are synthetic (the time does not depend on their
+    values)."""
+    import numpy as np
+    import torch
+    from boojum_cuda_amd import poseidon
+    n = 1 << log_n
+    poseidon.set_round_constants(np.random.default_rng(0).integers(0, 0xFFFFFFFF00000001, 360, dtype=np.uint64))
+    g = torch.Generator(device="cuda")
+    g.manual_seed(9)
+    vals = uniform_field(cols * n, g)
+    tree = torch.empty(8 * n, dtype=torch.int64, device="cuda")
+    leaves, nodes = tree[:4 * n], tree[4 * n:]
+    stream = torch.cuda.current_stream()
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2 * reps)]
+        for r in range(reps):
+            ev[2 * r].record(stream)
+            fn()
+            ev[2 * r + 1].record(stream)
+        torch.cuda.synchronize()
+        return sum(ev[2 * r].elapsed_time(ev[2 * r + 1]) for r in range(reps)) / reps
+
+    ms_leaves = timed(lambda: poseidon.build_merkle_tree_leaves(vals, leaves, 0))
+    ms_nodes = timed(lambda: poseidon.build_merkle_tree_nodes(leaves, nodes, log_n))
+    perms = n * ((cols + 7) // 8)
+    del vals
+    return {"workload": f"Merkle tree over {cols} columns x 2^{log_n} rows (one leaf per row)",
+            "ms_leaves": ms_leaves, "ms_nodes": ms_nodes, "ms_tree": ms_leaves + ms_nodes,
+            "leaf_permutations_per_s": perms / (ms_leaves * 1e-3),
+            "node_permutations_per_s": (n - 1) / (ms_nodes * 1e-3),
+            "leaf_input_GB/s": 8 * cols * n / (ms_leaves * 1e-3) / 1e9,
+            "round_constants": "synthetic"}
+
+
 def bench_latency(reps=200):
     """Config 1: one column of 2^16, forward natural -> bit-reversed, out of
     place.  Reports the GPU time per call (HIP events, back to back), the
@@ -396,6 +437,7 @@ def main():
     ap.add_argument("--no-ops", action="store_true", help="skip the bit-reverse / powers / omega-shift measurement")
     ap.add_argument("--no-bary", action="store_true", help="skip the barycentric-evaluation measurement")
     ap.add_argument("--no-latency", action="store_true", help="skip the single-column 2^16 latency measurement")
+    ap.add_argument("--no-poseidon", action="store_true", help="skip the Poseidon Merkle-tree measurement")
     ap.add_argument("--serve-requests", type=int, default=100)
     ap.add_argument("--serve-rate", type=float, default=25.0, help="offered load, requests per second")
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
@@ -493,6 +535,10 @@ def main():
         torch.cuda.empty_cache()
         extras["barycentric"] = bench_barycentric(log_n, cols, 5)
         progress(f"barycentric done: base_at_ext {extras['barycentric']['base_at_ext']['GB/s']:.0f} GB/s")
+    if not args.no_poseidon:
+        torch.cuda.empty_cache()
+        extras["poseidon"] = bench_poseidon(log_n, 128, 3)
+        progress(f"poseidon done: tree {extras['poseidon']['ms_tree']:.2f} ms")
     if not args.no_serve:
         torch.cuda.empty_cache()
         extras["serve"] = bench_serve(args.serve_requests, args.serve_rate, int(args.serve_budget_gb * 1e9))

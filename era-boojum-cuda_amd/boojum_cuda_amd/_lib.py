@@ -48,6 +48,12 @@ SIGNATURES = {
     "bc_barycentric_precompute": (c_int, [c_bool, c_vp, c_vp, c_u64, c_vp, c_u64, c_bool, c_vp]),
     "bc_batch_eval_temp_sizes": (c_int, [c_u32, c_u64, c_u32, ctypes.POINTER(c_size), ctypes.POINTER(c_size)]),
     "bc_batch_eval": (c_int, [c_u32, c_vp, c_u64, c_u32, c_vp, c_vp, c_size, c_vp, c_size, c_vp, c_vp]),
+    "bc_poseidon_set_round_constants": (c_int, [c_vp]),
+    "bc_poseidon_fused_table": (c_int, [c_vp, c_vp, c_u64]),
+    "bc_poseidon_leaves": (c_int, [c_vp, c_u64, c_vp, c_u64, c_u32, c_bool, c_bool, c_vp]),
+    "bc_poseidon_nodes": (c_int, [c_vp, c_u64, c_vp, c_u64, c_vp]),
+    "bc_poseidon_build_merkle_tree_nodes": (c_int, [c_vp, c_u64, c_vp, c_u64, c_u32, c_vp]),
+    "bc_poseidon_build_merkle_tree": (c_int, [c_vp, c_u64, c_vp, c_u64, c_u32, c_u32, c_vp]),
     "bc_fill_seeded": (c_int, [c_vp, c_u32, c_u32, c_u64, c_u64, c_u32, c_vp]),
     "bc_server_create": (c_int, [c_vp, ctypes.POINTER(c_vp)]),
     "bc_server_submit": (c_int, [c_vp, c_vp, ctypes.POINTER(c_u64)]),

@@ -1,0 +1,420 @@
+// Poseidon (Goldilocks, width 12 = rate 8 + capacity 4, 8 full and 22 partial
+// rounds, S-box x^7) Merkle-tree hashing: leaf hashes of column-major
+// matrices, node hashes of digest pairs, and whole trees.
+//
+// Reference: src/poseidon.rs:129-328 (argument checks, tree layout) and the
+// single-thread kernels native/poseidon_single_thread.cu:14-48,
+// native/poseidon_single_thread.cuh:18-145, native/poseidon_utils.cuh:8-61.
+//
+// Round constants.  The reference compiles boojum's 360 textbook round
+// constants into its kernels and runs the permutation in a fused form derived
+// from them at build time (build/poseidon_constants.rs).  Here the caller
+// supplies the 360 constants once (bc_poseidon_set_round_constants); the
+// library derives the same fused form on the host and keeps it in a
+// per-device table that every hashing kernel reads through the scalar cache:
+//   - rounds 0-2 and 27-29: textbook (constants, S-box on all 12, MDS);
+//   - round 3: its constants and S-box, then the fused constant layer and one
+//     dense 12 x 12 matrix (round 3's MDS times the first partial round's
+//     dense factor);
+//   - 22 partial rounds: S-box on element 0 plus one fused constant, then a
+//     sparse matrix (first row and first column plus the identity);
+//   - round 26: S-box and MDS only (its constants were moved into the
+//     partial rounds).
+// Derivation: each partial round's MDS is N = B A with A = diag(1, N^)
+// (commutes with the one-element S-box layer) and B = [[1, n^T N^-1],
+// [w, I]] sparse; the round before then has N = A M.  Constants move back
+// through M^-1: element 0 lands after the previous round's S-box, elements
+// 1..11 pass that S-box untouched.
+//
+// One thread hashes one leaf or node: the permutation is a serial chain of
+// field products with no data shared between hashes, and 12 state words fit
+// in registers at full occupancy.  Bound: VALU issue (about 25k VALU per
+// permutation against 64 bytes absorbed).
+#include "../../include/boojum_ntt.h"
+#include "gl_field.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <string.h>
+
+#include <mutex>
+#include <vector>
+
+namespace bc {
+namespace {
+
+constexpr int PT = 12, RATE = 8, CAP = 4, ROUNDS = 30, HALF_FULL = 4, PARTIAL = 22;
+constexpr unsigned MDS_EXPS[PT] = {0, 0, 1, 0, 3, 5, 1, 8, 12, 3, 16, 10};
+
+// Per-device table (canonical values).
+struct PosTable {
+  uint64_t rc_full[7][PT];  // textbook constants of rounds 0-3 and 27-29
+  uint64_t rcf[PT];         // fused constant layer after round 3's S-box
+  uint64_t dense[PT][PT];   // round 3's fused dense matrix
+  uint64_t sbox_c[PARTIAL]; // constant after each partial round's S-box
+  uint64_t vs[PARTIAL][PT - 1];     // first row (beyond the 1 at [0][0])
+  uint64_t w_hats[PARTIAL][PT - 1]; // first column (below [0][0])
+};
+
+__device__ PosTable pos_table;
+
+__device__ __forceinline__ uint64_t sbox(uint64_t x) {
+  const uint64_t x2 = gl::mul(x, x);
+  const uint64_t x3 = gl::mul(x2, x);
+  const uint64_t x4 = gl::mul(x2, x2);
+  return gl::mul(x3, x4);
+}
+
+// y_r = sum_i 2^e_i x_((r + i) mod 12): the shifted words summed into 96
+// bits (12 terms below 2^80) and reduced once.
+__device__ __forceinline__ void mds(uint64_t (&s)[PT]) {
+  uint64_t y[PT];
+#pragma unroll
+  for (int r = 0; r < PT; ++r) {
+    uint64_t lo = 0;
+    uint32_t hi = 0;
+#pragma unroll
+    for (int i = 0; i < PT; ++i) {
+      const uint64_t v = s[(r + i) % PT];
+      const unsigned e = MDS_EXPS[i];
+      const uint64_t t = e ? v << e : v;
+      lo += t;
+      hi += (e ? (uint32_t)(v >> (64 - e)) : 0u) + (lo < t ? 1u : 0u);
+    }
+    y[r] = gl::reduce_limbs((uint32_t)lo, (uint32_t)(lo >> 32), hi, 0u);
+  }
+#pragma unroll
+  for (int r = 0; r < PT; ++r) s[r] = y[r];
+}
+
+__device__ __forceinline__ void full_round(uint64_t (&s)[PT], const uint64_t *rc) {
+#pragma unroll
+  for (int i = 0; i < PT; ++i) s[i] = sbox(rc ? gl::add(s[i], rc[i]) : s[i]);
+}
+
+__device__ __forceinline__ void permute(uint64_t (&s)[PT]) {
+  const PosTable &t = pos_table;
+#pragma unroll 1
+  for (int r = 0; r < HALF_FULL - 1; ++r) {
+    full_round(s, t.rc_full[r]);
+    mds(s);
+  }
+  full_round(s, t.rc_full[HALF_FULL - 1]);
+  {
+    uint64_t v[PT];
+#pragma unroll
+    for (int i = 0; i < PT; ++i) v[i] = gl::add(s[i], t.rcf[i]);
+#pragma unroll
+    for (int r = 0; r < PT; ++r) {
+      gl::Acc acc;
+#pragma unroll
+      for (int c = 0; c < PT; ++c) gl::mac(acc, v[c], t.dense[r][c]);
+      s[r] = gl::acc_reduce(acc);
+    }
+  }
+#pragma unroll 1
+  for (int k = 0; k < PARTIAL; ++k) {
+    const uint64_t s0 = gl::add(sbox(s[0]), t.sbox_c[k]);
+    gl::Acc acc;
+    acc.l0 = (uint32_t)s0;
+    acc.l1 = (uint32_t)(s0 >> 32);
+#pragma unroll
+    for (int i = 1; i < PT; ++i) gl::mac(acc, s[i], t.vs[k][i - 1]);
+#pragma unroll
+    for (int i = 1; i < PT; ++i) s[i] = gl::add(gl::mul(s0, t.w_hats[k][i - 1]), s[i]);
+    s[0] = gl::acc_reduce(acc);
+  }
+  full_round(s, nullptr);
+  mds(s);
+#pragma unroll 1
+  for (int r = 0; r < 3; ++r) {
+    full_round(s, t.rc_full[HALF_FULL + r]);
+    mds(s);
+  }
+}
+
+// native/poseidon_utils.cuh:8-42: leaf g absorbs the values (row r, column c)
+// at values[g rows + r + c rows count] in the order c rows + r, 8 at a time
+// into state[0..8] (the last chunk zero-padded), permuting after each chunk.
+__global__ __launch_bounds__(256) void poseidon_leaves_kernel(const uint64_t *__restrict__ values,
+                                                              uint64_t *__restrict__ results, uint32_t rows,
+                                                              uint32_t cols, uint32_t count, bool load_intermediate,
+                                                              bool store_intermediate) {
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= count) return;
+  uint64_t s[PT];
+#pragma unroll
+  for (int i = 0; i < PT; ++i) s[i] = 0;
+  if (load_intermediate) {
+#pragma unroll
+    for (int i = 0; i < CAP; ++i) s[RATE + i] = gl::canon(results[g + (uint64_t)i * count]);
+  }
+  const uint64_t col_stride = (uint64_t)rows * count;
+  const uint64_t *base = values + (uint64_t)g * rows;
+  const uint64_t total = (uint64_t)rows * cols;
+  uint32_t r = 0, c = 0;  // position of `off` = c rows + r
+  for (uint64_t off = 0; off < total; off += RATE) {
+#pragma unroll
+    for (int i = 0; i < RATE; ++i) {
+      s[i] = c < cols ? gl::canon(base[r + c * col_stride]) : 0ull;
+      if (++r == rows) {
+        r = 0;
+        ++c;
+      }
+    }
+    permute(s);
+  }
+#pragma unroll
+  for (int i = 0; i < CAP; ++i) results[g + (uint64_t)i * count] = store_intermediate ? s[RATE + i] : s[i];
+}
+
+// native/poseidon_utils.cuh:44-61: node g hashes children 2g, 2g + 1 of the
+// layer below (4 columns of 2 count digests): state[i] = left[i],
+// state[4 + i] = right[i], capacity zero.
+__global__ __launch_bounds__(256) void poseidon_nodes_kernel(const uint64_t *__restrict__ values,
+                                                             uint64_t *__restrict__ results, uint32_t count) {
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= count) return;
+  uint64_t s[PT];
+#pragma unroll
+  for (int i = 0; i < CAP; ++i) {
+    const uint64_t *pair = values + 2 * ((uint64_t)g + (uint64_t)i * count);
+    s[i] = gl::canon(pair[0]);
+    s[CAP + i] = gl::canon(pair[1]);
+    s[RATE + i] = 0;
+  }
+  permute(s);
+#pragma unroll
+  for (int i = 0; i < CAP; ++i) results[g + (uint64_t)i * count] = s[i];
+}
+
+// ---- host: the fused form ---------------------------------------------------
+
+using Mat = std::vector<uint64_t>;  // row-major n x n
+
+uint64_t mds_entry(int r, int c) { return 1ull << MDS_EXPS[((c - r) % PT + PT) % PT]; }
+
+bool invert(Mat a, int n, Mat &out) {
+  out.assign((size_t)n * n, 0);
+  for (int i = 0; i < n; ++i) out[(size_t)i * n + i] = 1;
+  for (int col = 0; col < n; ++col) {
+    int piv = -1;
+    for (int r = col; r < n && piv < 0; ++r)
+      if (a[(size_t)r * n + col]) piv = r;
+    if (piv < 0) return false;
+    for (int j = 0; j < n; ++j) {
+      std::swap(a[(size_t)col * n + j], a[(size_t)piv * n + j]);
+      std::swap(out[(size_t)col * n + j], out[(size_t)piv * n + j]);
+    }
+    const uint64_t iv = gl::inv(a[(size_t)col * n + col]);
+    for (int j = 0; j < n; ++j) {
+      a[(size_t)col * n + j] = gl::mul(a[(size_t)col * n + j], iv);
+      out[(size_t)col * n + j] = gl::mul(out[(size_t)col * n + j], iv);
+    }
+    for (int r = 0; r < n; ++r) {
+      const uint64_t f = a[(size_t)r * n + col];
+      if (r == col || !f) continue;
+      for (int j = 0; j < n; ++j) {
+        a[(size_t)r * n + j] = gl::sub(a[(size_t)r * n + j], gl::mul(f, a[(size_t)col * n + j]));
+        out[(size_t)r * n + j] = gl::sub(out[(size_t)r * n + j], gl::mul(f, out[(size_t)col * n + j]));
+      }
+    }
+  }
+  return true;
+}
+
+void mat_vec(const Mat &m, const uint64_t *v, uint64_t *out) {
+  for (int r = 0; r < PT; ++r) {
+    uint64_t acc = 0;
+    for (int c = 0; c < PT; ++c) acc = gl::add(acc, gl::mul(m[(size_t)r * PT + c], v[c]));
+    out[r] = acc;
+  }
+}
+
+bool derive(const uint64_t *rc_in, PosTable &t) {
+  uint64_t rc[ROUNDS * PT];
+  for (int i = 0; i < ROUNDS * PT; ++i) rc[i] = gl::canon(rc_in[i]);
+  Mat m((size_t)PT * PT), minv;
+  for (int r = 0; r < PT; ++r)
+    for (int c = 0; c < PT; ++c) m[(size_t)r * PT + c] = mds_entry(r, c);
+  if (!invert(m, PT, minv)) return false;
+  for (int r = 0; r < HALF_FULL; ++r)
+    for (int i = 0; i < PT; ++i) t.rc_full[r][i] = rc[r * PT + i];
+  for (int r = 0; r < 3; ++r)
+    for (int i = 0; i < PT; ++i) t.rc_full[HALF_FULL + r][i] = rc[(HALF_FULL + PARTIAL + 1 + r) * PT + i];
+  // constants of round 26, then of each partial round, moved backwards
+  uint64_t cur[PT], e[PT];
+  for (int i = 0; i < PT; ++i) cur[i] = rc[(HALF_FULL + PARTIAL) * PT + i];
+  for (int k = HALF_FULL + PARTIAL - 1; k >= HALF_FULL; --k) {
+    mat_vec(minv, cur, e);
+    t.sbox_c[k - HALF_FULL] = e[0];
+    for (int i = 0; i < PT; ++i) cur[i] = i ? gl::add(rc[k * PT + i], e[i]) : rc[k * PT + i];
+  }
+  mat_vec(minv, cur, t.rcf);
+  // matrices
+  Mat n = m;
+  for (int k = HALF_FULL + PARTIAL - 1; k >= HALF_FULL; --k) {
+    const int pr = k - HALF_FULL;
+    Mat nh((size_t)(PT - 1) * (PT - 1)), nhinv;
+    for (int i = 0; i < PT - 1; ++i)
+      for (int j = 0; j < PT - 1; ++j) nh[(size_t)i * (PT - 1) + j] = n[(size_t)(i + 1) * PT + j + 1];
+    if (!invert(nh, PT - 1, nhinv)) return false;
+    for (int j = 0; j < PT - 1; ++j) {
+      uint64_t acc = 0;
+      for (int i = 0; i < PT - 1; ++i) acc = gl::add(acc, gl::mul(n[i + 1], nhinv[(size_t)i * (PT - 1) + j]));
+      t.vs[pr][j] = acc;
+      t.w_hats[pr][j] = n[(size_t)(j + 1) * PT];
+    }
+    Mat next((size_t)PT * PT);
+    for (int c = 0; c < PT; ++c) next[c] = m[c];
+    for (int r = 1; r < PT; ++r)
+      for (int c = 0; c < PT; ++c) {
+        uint64_t acc = 0;
+        for (int i = 1; i < PT; ++i)
+          acc = gl::add(acc, gl::mul(nh[(size_t)(r - 1) * (PT - 1) + i - 1], m[(size_t)i * PT + c]));
+        next[(size_t)r * PT + c] = acc;
+      }
+    n.swap(next);
+  }
+  for (int r = 0; r < PT; ++r)
+    for (int c = 0; c < PT; ++c) t.dense[r][c] = n[(size_t)r * PT + c];
+  return true;
+}
+
+constexpr int MAX_DEVICES = 64;
+std::mutex pos_mu;
+bool pos_ready[MAX_DEVICES];
+
+int require_table() {
+  int dev = 0;
+  const hipError_t err = hipGetDevice(&dev);
+  if (err != hipSuccess) return err;
+  if (dev < 0 || dev >= MAX_DEVICES) return BC_ERROR_INVALID_VALUE;
+  std::lock_guard<std::mutex> lk(pos_mu);
+  return pos_ready[dev] ? BC_SUCCESS : BC_ERROR_NOT_INITIALIZED;
+}
+
+int launch_leaves(const uint64_t *values, uint64_t *results, uint32_t rows, uint32_t cols, uint32_t count, bool load,
+                  bool store, hipStream_t s) {
+  if (count == 0) return BC_SUCCESS;
+  poseidon_leaves_kernel<<<dim3((count + 255) / 256), dim3(256), 0, s>>>(values, results, rows, cols, count, load,
+                                                                        store);
+  return hipGetLastError();
+}
+
+int launch_nodes(const uint64_t *values, uint64_t *results, uint32_t count, hipStream_t s) {
+  if (count == 0) return BC_SUCCESS;
+  poseidon_nodes_kernel<<<dim3((count + 255) / 256), dim3(256), 0, s>>>(values, results, count);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+size_t poseidon_table_words() { return sizeof(PosTable) / sizeof(uint64_t); }
+
+}  // namespace bc
+
+extern "C" int bc_poseidon_set_round_constants(const uint64_t *round_constants) {
+  using namespace bc;
+  if (!round_constants) return BC_ERROR_INVALID_VALUE;
+  int dev = 0;
+  hipError_t err = hipGetDevice(&dev);
+  if (err != hipSuccess) return err;
+  if (dev < 0 || dev >= MAX_DEVICES) return BC_ERROR_INVALID_VALUE;
+  PosTable t{};
+  if (!derive(round_constants, t)) return BC_ERROR_INVALID_VALUE;
+  std::lock_guard<std::mutex> lk(pos_mu);
+  err = hipMemcpyToSymbol(HIP_SYMBOL(pos_table), &t, sizeof t, 0, hipMemcpyHostToDevice);
+  if (err != hipSuccess) return err;
+  err = hipDeviceSynchronize();
+  if (err != hipSuccess) return err;
+  pos_ready[dev] = true;
+  return BC_SUCCESS;
+}
+
+extern "C" int bc_poseidon_fused_table(const uint64_t *round_constants, uint64_t *out, uint64_t out_words) {
+  using namespace bc;
+  if (!round_constants || !out || out_words < poseidon_table_words()) return BC_ERROR_INVALID_VALUE;
+  PosTable t{};
+  if (!derive(round_constants, t)) return BC_ERROR_INVALID_VALUE;
+  memcpy(out, &t, sizeof t);
+  return BC_SUCCESS;
+}
+
+extern "C" int bc_poseidon_leaves(const uint64_t *values, uint64_t values_len, uint64_t *results,
+                                  uint64_t results_len, uint32_t log_rows_per_hash, bool load_intermediate,
+                                  bool store_intermediate, void *stream) {
+  using namespace bc;
+  // src/poseidon.rs:139-155
+  if (log_rows_per_hash >= 32 || results_len % CAP != 0) return BC_ERROR_INVALID_VALUE;
+  const uint64_t count = results_len / CAP;
+  if (count == 0) return values_len == 0 ? BC_SUCCESS : BC_ERROR_INVALID_VALUE;
+  if (!values || !results || count > 0xFFFFFFFFull) return BC_ERROR_INVALID_VALUE;
+  const uint64_t rows = 1ull << log_rows_per_hash;
+  if (values_len % (count * rows) != 0) return BC_ERROR_INVALID_VALUE;
+  const uint64_t cols = values_len / (count * rows);
+  if (cols > 0xFFFFFFFFull || rows * cols > 0xFFFFFFFFull) return BC_ERROR_INVALID_VALUE;
+  if (store_intermediate && (rows * cols) % RATE != 0) return BC_ERROR_INVALID_VALUE;
+  const int rc = require_table();
+  if (rc != BC_SUCCESS) return rc;
+  return launch_leaves(values, results, (uint32_t)rows, (uint32_t)cols, (uint32_t)count, load_intermediate,
+                       store_intermediate, (hipStream_t)stream);
+}
+
+extern "C" int bc_poseidon_nodes(const uint64_t *values, uint64_t values_len, uint64_t *results,
+                                 uint64_t results_len, void *stream) {
+  using namespace bc;
+  // src/poseidon.rs:224-232
+  if (values_len % RATE != 0 || results_len % CAP != 0 || values_len != 2 * results_len)
+    return BC_ERROR_INVALID_VALUE;
+  const uint64_t count = results_len / CAP;
+  if (count == 0) return BC_SUCCESS;
+  if (!values || !results || count > 0xFFFFFFFFull) return BC_ERROR_INVALID_VALUE;
+  const int rc = require_table();
+  if (rc != BC_SUCCESS) return rc;
+  return launch_nodes(values, results, (uint32_t)count, (hipStream_t)stream);
+}
+
+extern "C" int bc_poseidon_build_merkle_tree_nodes(const uint64_t *values, uint64_t values_len, uint64_t *results,
+                                                   uint64_t results_len, uint32_t layers_count, void *stream) {
+  using namespace bc;
+  // src/poseidon.rs:262-286: each layer halves; its nodes fill the first half
+  // of what remains of `results`
+  if (layers_count == 0) return BC_SUCCESS;
+  if (values_len % RATE != 0 || values_len != results_len) return BC_ERROR_INVALID_VALUE;
+  const uint64_t leaves = values_len / RATE;
+  if (leaves == 0 || (leaves & (leaves - 1)) != 0) return BC_ERROR_INVALID_VALUE;
+  if (layers_count > 64 || (layers_count > 1 && (values_len >> (layers_count - 1)) < RATE))
+    return BC_ERROR_INVALID_VALUE;
+  const int rc = require_table();
+  if (rc != BC_SUCCESS) return rc;
+  const uint64_t *in = values;
+  uint64_t len = values_len;
+  uint64_t *out = results;
+  for (uint32_t l = 0; l < layers_count; ++l) {
+    const uint64_t half = len / 2;
+    const int e = bc_poseidon_nodes(in, len, out, half, stream);
+    if (e != BC_SUCCESS) return e;
+    in = out;
+    out += half;
+    len = half;
+  }
+  return BC_SUCCESS;
+}
+
+extern "C" int bc_poseidon_build_merkle_tree(const uint64_t *values, uint64_t values_len, uint64_t *results,
+                                             uint64_t results_len, uint32_t log_rows_per_hash,
+                                             uint32_t layers_count, void *stream) {
+  using namespace bc;
+  // src/poseidon.rs:311-328: the leaf digests fill the first half of
+  // `results`, the node layers the rest
+  if (layers_count == 0 || results_len % (2 * CAP) != 0) return BC_ERROR_INVALID_VALUE;
+  const uint64_t leaves = results_len / (2 * CAP);
+  if (leaves == 0 || layers_count - 1 >= 64 || (1ull << (layers_count - 1)) > leaves ||
+      values_len % leaves != 0)
+    return BC_ERROR_INVALID_VALUE;
+  const uint64_t half = results_len / 2;
+  int e = bc_poseidon_leaves(values, values_len, results, half, log_rows_per_hash, false, false, stream);
+  if (e != BC_SUCCESS) return e;
+  return bc_poseidon_build_merkle_tree_nodes(results, half, results + half, half, layers_count - 1, stream);
+}

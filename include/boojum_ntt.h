@@ -3,7 +3,8 @@
  * (libboojum_ntt.so).  This is the drop-in boundary for boojum-cuda's `ntt`
  * and `context` modules and for the functions either side of the transform
  * (`ops_complex` bit reversal, domain powers and omega shift; `barycentric`
- * evaluation): a Rust FFI crate (or any C caller) binds exactly these symbols
+ * evaluation; `poseidon` Merkle-tree hashing of the LDE output): a Rust FFI
+ * crate (or any C caller) binds exactly these symbols
  * in place of the CUDA kernels that era_cudart launched.
  *
  * Conventions (same as the reference):
@@ -227,6 +228,57 @@ int bc_batch_eval_temp_sizes(uint32_t kind, uint64_t stride, uint32_t num_polys,
 int bc_batch_eval(uint32_t kind, const uint64_t *ys, uint64_t stride, uint32_t num_polys, const uint64_t *coeffs,
                   uint64_t *temp_partial, size_t temp_partial_elems, void *temp_final, size_t temp_final_bytes,
                   uint64_t *evals, void *stream);
+
+/* ---- Poseidon Merkle trees (reference: src/poseidon.rs:129-328) ----
+ * Poseidon over Goldilocks: width 12 (rate 8, capacity 4), 8 full and 22
+ * partial rounds, S-box x^7, MDS = circulant of 2^{0,0,1,0,3,5,1,8,12,3,16,10}
+ * (native/poseidon_constants_template.cuh:26).  Digests are 4 elements,
+ * stored as 4 columns of `count` (element i of digest g at results[g + i *
+ * count]).  Outputs are canonical; raw inputs are read mod p. */
+
+/* Replaces the constants the reference compiles in from boojum
+ * (build/poseidon_constants.rs:66-75): round_constants = the 360 textbook
+ * round constants (30 rounds x 12, round-major; boojum's
+ * poseidon_goldilocks_params::ALL_ROUND_CONSTANTS).  Derives the fused form
+ * the kernels run (the reference's poseidon_constants_template.cuh tables)
+ * and stores it on the current device; synchronous.  Every hashing call on a
+ * device returns BC_ERROR_NOT_INITIALIZED until this has been called there. */
+int bc_poseidon_set_round_constants(const uint64_t *round_constants);
+
+/* Host only (no device call): the fused table derived from 360 constants, as
+ * out_words >= 746 u64: textbook constants of rounds 0-3 and 27-29 (84), the
+ * fused constant layer (12), the dense matrix (144, row-major), the fused
+ * S-box constants (22), VS (22 x 11) and W_HATS (22 x 11). */
+int bc_poseidon_fused_table(const uint64_t *round_constants, uint64_t *out, uint64_t out_words);
+
+/* launch_leaves_kernel (src/poseidon.rs:129-170): count = results_len / 4
+ * leaves; leaf g hashes rows = 2^log_rows_per_hash consecutive rows of each of
+ * cols = values_len / (count rows) columns (value (r, c) of leaf g at
+ * values[g rows + r + c rows count]), absorbed 8 at a time in the order
+ * c rows + r, the last chunk zero-padded (native/poseidon_utils.cuh:8-42).
+ * load_intermediate: start from the capacity stored in results by an earlier
+ * call with store_intermediate, which stores the capacity instead of the
+ * digest and requires rows * cols to be a multiple of 8. */
+int bc_poseidon_leaves(const uint64_t *values, uint64_t values_len, uint64_t *results, uint64_t results_len,
+                       uint32_t log_rows_per_hash, bool load_intermediate, bool store_intermediate, void *stream);
+
+/* launch_single_thread_nodes_kernel (src/poseidon.rs:218-238): count =
+ * results_len / 4 nodes from a layer of 2 count digests (values_len =
+ * 2 results_len); node g hashes digests 2g and 2g + 1. */
+int bc_poseidon_nodes(const uint64_t *values, uint64_t values_len, uint64_t *results, uint64_t results_len,
+                      void *stream);
+
+/* build_merkle_tree_nodes (src/poseidon.rs:262-286): layers_count node layers
+ * above the layer in values (values_len = results_len, 8 2^k elements); layer
+ * l fills the first half of what layers 0..l-1 left of results. */
+int bc_poseidon_build_merkle_tree_nodes(const uint64_t *values, uint64_t values_len, uint64_t *results,
+                                        uint64_t results_len, uint32_t layers_count, void *stream);
+
+/* build_merkle_tree (src/poseidon.rs:311-328): leaves into the first half of
+ * results (results_len = 8 leaves), then layers_count - 1 node layers. */
+int bc_poseidon_build_merkle_tree(const uint64_t *values, uint64_t values_len, uint64_t *results,
+                                  uint64_t results_len, uint32_t log_rows_per_hash, uint32_t layers_count,
+                                  void *stream);
 
 /* Seeded inputs on the device: column c (c = 0..num_cols-1) at
  * out + c * stride gets 2^log_n u64 draws of the splitmix64 stream seeded with
