@@ -289,7 +289,8 @@ def bench_barycentric(log_n, cols, reps):
 
 
 def bench_poseidon(log_n, cols, reps):
-    """Poseidon Merkle tree over one LDE coset: `cols` columns x 2^log_n rows,
+    """Poseidon and Poseidon2 Merkle trees over one LDE coset: `cols` columns x
+    2^log_n rows,
     one leaf per row (cols / 8 permutations per leaf), then the 2^log_n - 1
     node hashes.  The permutation is VALU-bound; GB/s is the leaf input read
     once.  Round constants are synthetic (the time does not depend on their
@@ -317,16 +318,18 @@ def bench_poseidon(log_n, cols, reps):
         torch.cuda.synchronize()
         return sum(ev[2 * r].elapsed_time(ev[2 * r + 1]) for r in range(reps)) / reps
 
-    ms_leaves = timed(lambda: poseidon.build_merkle_tree_leaves(vals, leaves, 0))
-    ms_nodes = timed(lambda: poseidon.build_merkle_tree_nodes(leaves, nodes, log_n))
     perms = n * ((cols + 7) // 8)
+    out = {"workload": f"Merkle tree over {cols} columns x 2^{log_n} rows (one leaf per row)",
+           "round_constants": "synthetic"}
+    for p in (poseidon.POSEIDON, poseidon.POSEIDON2):
+        ms_leaves = timed(lambda: poseidon.build_merkle_tree_leaves(vals, leaves, 0, p=p))
+        ms_nodes = timed(lambda: poseidon.build_merkle_tree_nodes(leaves, nodes, log_n, p=p))
+        out[p] = {"ms_leaves": ms_leaves, "ms_nodes": ms_nodes, "ms_tree": ms_leaves + ms_nodes,
+                  "leaf_permutations_per_s": perms / (ms_leaves * 1e-3),
+                  "node_permutations_per_s": (n - 1) / (ms_nodes * 1e-3),
+                  "leaf_input_GB/s": 8 * cols * n / (ms_leaves * 1e-3) / 1e9}
     del vals
-    return {"workload": f"Merkle tree over {cols} columns x 2^{log_n} rows (one leaf per row)",
-            "ms_leaves": ms_leaves, "ms_nodes": ms_nodes, "ms_tree": ms_leaves + ms_nodes,
-            "leaf_permutations_per_s": perms / (ms_leaves * 1e-3),
-            "node_permutations_per_s": (n - 1) / (ms_nodes * 1e-3),
-            "leaf_input_GB/s": 8 * cols * n / (ms_leaves * 1e-3) / 1e9,
-            "round_constants": "synthetic"}
+    return out
 
 
 def bench_latency(reps=200):
@@ -538,7 +541,8 @@ def main():
     if not args.no_poseidon:
         torch.cuda.empty_cache()
         extras["poseidon"] = bench_poseidon(log_n, 128, 3)
-        progress(f"poseidon done: tree {extras['poseidon']['ms_tree']:.2f} ms")
+        progress(f"poseidon done: tree {extras['poseidon']['poseidon']['ms_tree']:.2f} ms, "
+                 f"poseidon2 {extras['poseidon']['poseidon2']['ms_tree']:.2f} ms")
     if not args.no_serve:
         torch.cuda.empty_cache()
         extras["serve"] = bench_serve(args.serve_requests, args.serve_rate, int(args.serve_budget_gb * 1e9))

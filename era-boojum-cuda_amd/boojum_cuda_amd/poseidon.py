@@ -1,12 +1,14 @@
-"""Poseidon Merkle-tree hashing, mirroring the reference's `poseidon` module
-(src/poseidon.rs:129-328) for its Poseidon permutation over the C ABI.
+"""Poseidon and Poseidon2 Merkle-tree hashing, mirroring the reference's
+`poseidon` module (src/poseidon.rs:129-328) over the C ABI.  Where the
+reference is generic over `P: PoseidonImpl`, these functions take
+`p=POSEIDON` or `p=POSEIDON2`.
 Tensors are HIP torch tensors of int64/uint64; digests are 4 elements, stored
 as 4 columns of `count` (element i of digest g at results[g + i * count]).
 
 The permutation's 360 round constants are boojum's
 (poseidon_goldilocks_params::ALL_ROUND_CONSTANTS), which the reference
 compiles in at build time; here they are handed to `set_round_constants`
-once per device before any hashing call."""
+once per device before any hashing call; both hashes read them."""
 import ctypes
 
 import numpy as np
@@ -17,7 +19,10 @@ RATE = 8
 CAPACITY = 4
 STATE_WIDTH = 12
 NUM_ROUND_CONSTANTS = 360
-FUSED_TABLE_WORDS = 746
+FUSED_TABLE_WORDS = 864
+
+POSEIDON = "poseidon"    # struct Poseidon (src/poseidon.rs)
+POSEIDON2 = "poseidon2"  # struct Poseidon2
 
 
 def _check(rc, what):
@@ -57,48 +62,53 @@ def fused_table(round_constants):
     _check(lib().bc_poseidon_fused_table(ptr, out.ctypes.data_as(ctypes.c_void_p), out.size), "fused_table")
     res, o = {"words": out}, 0
     for name, shape in (("rc_full", (7, 12)), ("rcf", (12,)), ("dense", (12, 12)), ("sbox_c", (22,)),
-                        ("vs", (22, 11)), ("w_hats", (22, 11))):
+                        ("vs", (22, 11)), ("w_hats", (22, 11)), ("p2_full", (8, 12)), ("p2_partial", (22,))):
         k = int(np.prod(shape))
         res[name] = out[o:o + k].reshape(shape)
         o += k
     return res
 
 
+def _fn(p, name):
+    assert p in (POSEIDON, POSEIDON2)
+    return getattr(lib(), f"bc_{p}_{name}")
+
+
 def launch_single_thread_leaves_kernel(values, results, log_rows_per_hash, load_intermediate=False,
-                                       store_intermediate=False, stream=None):
+                                       store_intermediate=False, stream=None, p=POSEIDON):
     """src/poseidon.rs:172-192 (argument checks of launch_leaves_kernel,
     src/poseidon.rs:129-170)."""
-    _check(lib().bc_poseidon_leaves(values.data_ptr(), values.numel(), results.data_ptr(), results.numel(),
-                                    log_rows_per_hash, bool(load_intermediate), bool(store_intermediate),
-                                    _stream(stream)), "launch_leaves_kernel")
+    _check(_fn(p, "leaves")(values.data_ptr(), values.numel(), results.data_ptr(), results.numel(),
+                            log_rows_per_hash, bool(load_intermediate), bool(store_intermediate),
+                            _stream(stream)), "launch_leaves_kernel")
 
 
-def launch_single_thread_nodes_kernel(values, results, stream=None):
+def launch_single_thread_nodes_kernel(values, results, stream=None, p=POSEIDON):
     """src/poseidon.rs:218-238."""
-    _check(lib().bc_poseidon_nodes(values.data_ptr(), values.numel(), results.data_ptr(), results.numel(),
-                                   _stream(stream)), "launch_nodes_kernel")
+    _check(_fn(p, "nodes")(values.data_ptr(), values.numel(), results.data_ptr(), results.numel(),
+                           _stream(stream)), "launch_nodes_kernel")
 
 
 def build_merkle_tree_leaves(values, results, log_rows_per_hash, load_intermediate=False,
-                             store_intermediate=False, stream=None):
+                             store_intermediate=False, stream=None, p=POSEIDON):
     """src/poseidon.rs:288-309."""
     assert results.numel() % CAPACITY == 0
     leaves = results.numel() // CAPACITY
     assert leaves and values.numel() % leaves == 0
     launch_single_thread_leaves_kernel(values, results, log_rows_per_hash, load_intermediate, store_intermediate,
-                                       stream)
+                                       stream, p)
 
 
-def build_merkle_tree_nodes(values, results, layers_count, stream=None):
+def build_merkle_tree_nodes(values, results, layers_count, stream=None, p=POSEIDON):
     """src/poseidon.rs:262-286."""
-    _check(lib().bc_poseidon_build_merkle_tree_nodes(values.data_ptr(), values.numel(), results.data_ptr(),
-                                                     results.numel(), layers_count, _stream(stream)),
+    _check(_fn(p, "build_merkle_tree_nodes")(values.data_ptr(), values.numel(), results.data_ptr(),
+                                             results.numel(), layers_count, _stream(stream)),
            "build_merkle_tree_nodes")
 
 
-def build_merkle_tree(values, results, log_rows_per_hash, stream=None, layers_count=1):
+def build_merkle_tree(values, results, log_rows_per_hash, stream=None, layers_count=1, p=POSEIDON):
     """src/poseidon.rs:311-328: results (8 leaves elements) holds the leaf
     digests in its first half and the layers_count - 1 node layers after."""
-    _check(lib().bc_poseidon_build_merkle_tree(values.data_ptr(), values.numel(), results.data_ptr(),
-                                               results.numel(), log_rows_per_hash, layers_count, _stream(stream)),
+    _check(_fn(p, "build_merkle_tree")(values.data_ptr(), values.numel(), results.data_ptr(),
+                                       results.numel(), log_rows_per_hash, layers_count, _stream(stream)),
            "build_merkle_tree")

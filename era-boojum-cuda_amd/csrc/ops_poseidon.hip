@@ -1,10 +1,12 @@
-// Poseidon (Goldilocks, width 12 = rate 8 + capacity 4, 8 full and 22 partial
-// rounds, S-box x^7) Merkle-tree hashing: leaf hashes of column-major
-// matrices, node hashes of digest pairs, and whole trees.
+// Poseidon and Poseidon2 (Goldilocks, width 12 = rate 8 + capacity 4, 8 full
+// and 22 partial rounds, S-box x^7) Merkle-tree hashing: leaf hashes of
+// column-major matrices, node hashes of digest pairs, and whole trees.
 //
 // Reference: src/poseidon.rs:129-328 (argument checks, tree layout) and the
 // single-thread kernels native/poseidon_single_thread.cu:14-48,
-// native/poseidon_single_thread.cuh:18-145, native/poseidon_utils.cuh:8-61.
+// native/poseidon_single_thread.cuh:18-145, native/poseidon2_single_thread.cu,
+// native/poseidon2_single_thread.cuh, native/poseidon_utils.cuh:8-61.  Both
+// hashes read the same 360 round constants.
 //
 // Round constants.  The reference compiles boojum's 360 textbook round
 // constants into its kernels and runs the permutation in a fused form derived
@@ -54,6 +56,10 @@ struct PosTable {
   uint64_t sbox_c[PARTIAL]; // constant after each partial round's S-box
   uint64_t vs[PARTIAL][PT - 1];     // first row (beyond the 1 at [0][0])
   uint64_t w_hats[PARTIAL][PT - 1]; // first column (below [0][0])
+  // Poseidon2 (textbook form): constants of the full rounds 0-3 and 26-29,
+  // and element 0's constant of each partial round
+  uint64_t p2_full[8][PT];
+  uint64_t p2_partial[PARTIAL];
 };
 
 __device__ PosTable pos_table;
@@ -133,9 +139,87 @@ __device__ __forceinline__ void permute(uint64_t (&s)[PT]) {
   }
 }
 
+// ---- Poseidon2 (native/poseidon2_single_thread.cu:7-24,
+// native/poseidon2_single_thread.cuh:12-66): external matrix M_E = circ(2 M4,
+// M4, M4) over the three 4-tiles, internal matrix M_I = J + diag(2^mu_i).
+constexpr unsigned P2_MU[PT] = {4, 14, 11, 8, 0, 5, 2, 9, 13, 6, 3, 12};
+
+// M4 = [[5,7,1,3],[4,6,1,1],[1,3,5,7],[1,1,4,6]] on one tile, by the
+// addition chain of eprint 2023/323 Appendix B (8 additions, 4 doublings).
+__device__ __forceinline__ void m4_tile(uint64_t *x) {
+  const uint64_t t0 = gl::add(x[0], x[1]);
+  const uint64_t t1 = gl::add(x[2], x[3]);
+  const uint64_t t2 = gl::add(gl::add(x[1], x[1]), t1);
+  const uint64_t t3 = gl::add(gl::add(x[3], x[3]), t0);
+  const uint64_t t1x2 = gl::add(t1, t1), t0x2 = gl::add(t0, t0);
+  const uint64_t t4 = gl::add(gl::add(t1x2, t1x2), t3);
+  const uint64_t t5 = gl::add(gl::add(t0x2, t0x2), t2);
+  x[0] = gl::add(t3, t5);
+  x[1] = t5;
+  x[2] = gl::add(t2, t4);
+  x[3] = t4;
+}
+
+__device__ __forceinline__ void m_ext(uint64_t (&s)[PT]) {
+  m4_tile(s);
+  m4_tile(s + 4);
+  m4_tile(s + 8);
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const uint64_t sum = gl::add(gl::add(s[j], s[4 + j]), s[8 + j]);
+    s[j] = gl::add(s[j], sum);
+    s[4 + j] = gl::add(s[4 + j], sum);
+    s[8 + j] = gl::add(s[8 + j], sum);
+  }
+}
+
+// y_i = sum + 2^mu_i x_i: the shifted word and the sum added in 96 bits,
+// one reduction.
+__device__ __forceinline__ void m_int(uint64_t (&s)[PT]) {
+  uint64_t sum = s[0];
+#pragma unroll
+  for (int i = 1; i < PT; ++i) sum = gl::add(sum, s[i]);
+#pragma unroll
+  for (int i = 0; i < PT; ++i) {
+    const unsigned e = P2_MU[i];
+    uint64_t lo = e ? s[i] << e : s[i];
+    uint32_t hi = e ? (uint32_t)(s[i] >> (64 - e)) : 0u;
+    lo += sum;
+    hi += lo < sum ? 1u : 0u;
+    s[i] = gl::reduce_limbs((uint32_t)lo, (uint32_t)(lo >> 32), hi, 0u);
+  }
+}
+
+__device__ __forceinline__ void permute2(uint64_t (&s)[PT]) {
+  const PosTable &t = pos_table;
+  m_ext(s);
+#pragma unroll 1
+  for (int r = 0; r < HALF_FULL; ++r) {
+    full_round(s, t.p2_full[r]);
+    m_ext(s);
+  }
+#pragma unroll 1
+  for (int k = 0; k < PARTIAL; ++k) {
+    s[0] = sbox(gl::add(s[0], t.p2_partial[k]));
+    m_int(s);
+  }
+#pragma unroll 1
+  for (int r = 0; r < HALF_FULL; ++r) {
+    full_round(s, t.p2_full[HALF_FULL + r]);
+    m_ext(s);
+  }
+}
+
+// Hash selector of the kernels: 0 Poseidon, 1 Poseidon2.
+template <int H> __device__ __forceinline__ void permute_h(uint64_t (&s)[PT]) {
+  if constexpr (H == 0) permute(s);
+  else permute2(s);
+}
+
 // native/poseidon_utils.cuh:8-42: leaf g absorbs the values (row r, column c)
 // at values[g rows + r + c rows count] in the order c rows + r, 8 at a time
 // into state[0..8] (the last chunk zero-padded), permuting after each chunk.
+template <int H>
 __global__ __launch_bounds__(256) void poseidon_leaves_kernel(const uint64_t *__restrict__ values,
                                                               uint64_t *__restrict__ results, uint32_t rows,
                                                               uint32_t cols, uint32_t count, bool load_intermediate,
@@ -162,7 +246,7 @@ __global__ __launch_bounds__(256) void poseidon_leaves_kernel(const uint64_t *__
         ++c;
       }
     }
-    permute(s);
+    permute_h<H>(s);
   }
 #pragma unroll
   for (int i = 0; i < CAP; ++i) results[g + (uint64_t)i * count] = store_intermediate ? s[RATE + i] : s[i];
@@ -171,6 +255,7 @@ __global__ __launch_bounds__(256) void poseidon_leaves_kernel(const uint64_t *__
 // native/poseidon_utils.cuh:44-61: node g hashes children 2g, 2g + 1 of the
 // layer below (4 columns of 2 count digests): state[i] = left[i],
 // state[4 + i] = right[i], capacity zero.
+template <int H>
 __global__ __launch_bounds__(256) void poseidon_nodes_kernel(const uint64_t *__restrict__ values,
                                                              uint64_t *__restrict__ results, uint32_t count) {
   const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
@@ -183,7 +268,7 @@ __global__ __launch_bounds__(256) void poseidon_nodes_kernel(const uint64_t *__r
     s[CAP + i] = gl::canon(pair[1]);
     s[RATE + i] = 0;
   }
-  permute(s);
+  permute_h<H>(s);
 #pragma unroll
   for (int i = 0; i < CAP; ++i) results[g + (uint64_t)i * count] = s[i];
 }
@@ -242,6 +327,12 @@ bool derive(const uint64_t *rc_in, PosTable &t) {
     for (int i = 0; i < PT; ++i) t.rc_full[r][i] = rc[r * PT + i];
   for (int r = 0; r < 3; ++r)
     for (int i = 0; i < PT; ++i) t.rc_full[HALF_FULL + r][i] = rc[(HALF_FULL + PARTIAL + 1 + r) * PT + i];
+  for (int r = 0; r < HALF_FULL; ++r)
+    for (int i = 0; i < PT; ++i) {
+      t.p2_full[r][i] = rc[r * PT + i];
+      t.p2_full[HALF_FULL + r][i] = rc[(HALF_FULL + PARTIAL + r) * PT + i];
+    }
+  for (int k = 0; k < PARTIAL; ++k) t.p2_partial[k] = rc[(HALF_FULL + k) * PT];
   // constants of round 26, then of each partial round, moved backwards
   uint64_t cur[PT], e[PT];
   for (int i = 0; i < PT; ++i) cur[i] = rc[(HALF_FULL + PARTIAL) * PT + i];
@@ -294,18 +385,97 @@ int require_table() {
   return pos_ready[dev] ? BC_SUCCESS : BC_ERROR_NOT_INITIALIZED;
 }
 
+template <int H>
 int launch_leaves(const uint64_t *values, uint64_t *results, uint32_t rows, uint32_t cols, uint32_t count, bool load,
                   bool store, hipStream_t s) {
   if (count == 0) return BC_SUCCESS;
-  poseidon_leaves_kernel<<<dim3((count + 255) / 256), dim3(256), 0, s>>>(values, results, rows, cols, count, load,
-                                                                        store);
+  poseidon_leaves_kernel<H><<<dim3((count + 255) / 256), dim3(256), 0, s>>>(values, results, rows, cols, count, load,
+                                                                           store);
   return hipGetLastError();
 }
 
-int launch_nodes(const uint64_t *values, uint64_t *results, uint32_t count, hipStream_t s) {
+template <int H> int launch_nodes(const uint64_t *values, uint64_t *results, uint32_t count, hipStream_t s) {
   if (count == 0) return BC_SUCCESS;
-  poseidon_nodes_kernel<<<dim3((count + 255) / 256), dim3(256), 0, s>>>(values, results, count);
+  poseidon_nodes_kernel<H><<<dim3((count + 255) / 256), dim3(256), 0, s>>>(values, results, count);
   return hipGetLastError();
+}
+
+// The entry points of both hashes share these (src/poseidon.rs is generic
+// over the hash in the same way).
+
+template <int H>
+int leaves_impl(const uint64_t *values, uint64_t values_len, uint64_t *results, uint64_t results_len,
+                uint32_t log_rows_per_hash, bool load_intermediate, bool store_intermediate, void *stream) {
+  // src/poseidon.rs:139-155
+  if (log_rows_per_hash >= 32 || results_len % CAP != 0) return BC_ERROR_INVALID_VALUE;
+  const uint64_t count = results_len / CAP;
+  if (count == 0) return values_len == 0 ? BC_SUCCESS : BC_ERROR_INVALID_VALUE;
+  if (!values || !results || count > 0xFFFFFFFFull) return BC_ERROR_INVALID_VALUE;
+  const uint64_t rows = 1ull << log_rows_per_hash;
+  if (values_len % (count * rows) != 0) return BC_ERROR_INVALID_VALUE;
+  const uint64_t cols = values_len / (count * rows);
+  if (cols > 0xFFFFFFFFull || rows * cols > 0xFFFFFFFFull) return BC_ERROR_INVALID_VALUE;
+  if (store_intermediate && (rows * cols) % RATE != 0) return BC_ERROR_INVALID_VALUE;
+  const int rc = require_table();
+  if (rc != BC_SUCCESS) return rc;
+  return launch_leaves<H>(values, results, (uint32_t)rows, (uint32_t)cols, (uint32_t)count, load_intermediate,
+                          store_intermediate, (hipStream_t)stream);
+}
+
+template <int H>
+int nodes_impl(const uint64_t *values, uint64_t values_len, uint64_t *results, uint64_t results_len, void *stream) {
+  // src/poseidon.rs:224-232
+  if (values_len % RATE != 0 || results_len % CAP != 0 || values_len != 2 * results_len)
+    return BC_ERROR_INVALID_VALUE;
+  const uint64_t count = results_len / CAP;
+  if (count == 0) return BC_SUCCESS;
+  if (!values || !results || count > 0xFFFFFFFFull) return BC_ERROR_INVALID_VALUE;
+  const int rc = require_table();
+  if (rc != BC_SUCCESS) return rc;
+  return launch_nodes<H>(values, results, (uint32_t)count, (hipStream_t)stream);
+}
+
+template <int H>
+int tree_nodes_impl(const uint64_t *values, uint64_t values_len, uint64_t *results, uint64_t results_len,
+                    uint32_t layers_count, void *stream) {
+  // src/poseidon.rs:262-286: each layer halves; its nodes fill the first half
+  // of what remains of `results`
+  if (layers_count == 0) return BC_SUCCESS;
+  if (values_len % RATE != 0 || values_len != results_len) return BC_ERROR_INVALID_VALUE;
+  const uint64_t leaves = values_len / RATE;
+  if (leaves == 0 || (leaves & (leaves - 1)) != 0) return BC_ERROR_INVALID_VALUE;
+  if (layers_count > 64 || (layers_count > 1 && (values_len >> (layers_count - 1)) < RATE))
+    return BC_ERROR_INVALID_VALUE;
+  const int rc = require_table();
+  if (rc != BC_SUCCESS) return rc;
+  const uint64_t *in = values;
+  uint64_t len = values_len;
+  uint64_t *out = results;
+  for (uint32_t l = 0; l < layers_count; ++l) {
+    const uint64_t half = len / 2;
+    const int e = nodes_impl<H>(in, len, out, half, stream);
+    if (e != BC_SUCCESS) return e;
+    in = out;
+    out += half;
+    len = half;
+  }
+  return BC_SUCCESS;
+}
+
+template <int H>
+int tree_impl(const uint64_t *values, uint64_t values_len, uint64_t *results, uint64_t results_len,
+              uint32_t log_rows_per_hash, uint32_t layers_count, void *stream) {
+  // src/poseidon.rs:311-328: the leaf digests fill the first half of
+  // `results`, the node layers the rest
+  if (layers_count == 0 || results_len % (2 * CAP) != 0) return BC_ERROR_INVALID_VALUE;
+  const uint64_t leaves = results_len / (2 * CAP);
+  if (leaves == 0 || layers_count - 1 >= 64 || (1ull << (layers_count - 1)) > leaves ||
+      values_len % leaves != 0)
+    return BC_ERROR_INVALID_VALUE;
+  const uint64_t half = results_len / 2;
+  const int e = leaves_impl<H>(values, values_len, results, half, log_rows_per_hash, false, false, stream);
+  if (e != BC_SUCCESS) return e;
+  return tree_nodes_impl<H>(results, half, results + half, half, layers_count - 1, stream);
 }
 
 }  // namespace
@@ -344,77 +514,45 @@ extern "C" int bc_poseidon_fused_table(const uint64_t *round_constants, uint64_t
 extern "C" int bc_poseidon_leaves(const uint64_t *values, uint64_t values_len, uint64_t *results,
                                   uint64_t results_len, uint32_t log_rows_per_hash, bool load_intermediate,
                                   bool store_intermediate, void *stream) {
-  using namespace bc;
-  // src/poseidon.rs:139-155
-  if (log_rows_per_hash >= 32 || results_len % CAP != 0) return BC_ERROR_INVALID_VALUE;
-  const uint64_t count = results_len / CAP;
-  if (count == 0) return values_len == 0 ? BC_SUCCESS : BC_ERROR_INVALID_VALUE;
-  if (!values || !results || count > 0xFFFFFFFFull) return BC_ERROR_INVALID_VALUE;
-  const uint64_t rows = 1ull << log_rows_per_hash;
-  if (values_len % (count * rows) != 0) return BC_ERROR_INVALID_VALUE;
-  const uint64_t cols = values_len / (count * rows);
-  if (cols > 0xFFFFFFFFull || rows * cols > 0xFFFFFFFFull) return BC_ERROR_INVALID_VALUE;
-  if (store_intermediate && (rows * cols) % RATE != 0) return BC_ERROR_INVALID_VALUE;
-  const int rc = require_table();
-  if (rc != BC_SUCCESS) return rc;
-  return launch_leaves(values, results, (uint32_t)rows, (uint32_t)cols, (uint32_t)count, load_intermediate,
-                       store_intermediate, (hipStream_t)stream);
+  return bc::leaves_impl<0>(values, values_len, results, results_len, log_rows_per_hash, load_intermediate,
+                            store_intermediate, stream);
 }
 
 extern "C" int bc_poseidon_nodes(const uint64_t *values, uint64_t values_len, uint64_t *results,
                                  uint64_t results_len, void *stream) {
-  using namespace bc;
-  // src/poseidon.rs:224-232
-  if (values_len % RATE != 0 || results_len % CAP != 0 || values_len != 2 * results_len)
-    return BC_ERROR_INVALID_VALUE;
-  const uint64_t count = results_len / CAP;
-  if (count == 0) return BC_SUCCESS;
-  if (!values || !results || count > 0xFFFFFFFFull) return BC_ERROR_INVALID_VALUE;
-  const int rc = require_table();
-  if (rc != BC_SUCCESS) return rc;
-  return launch_nodes(values, results, (uint32_t)count, (hipStream_t)stream);
+  return bc::nodes_impl<0>(values, values_len, results, results_len, stream);
 }
 
 extern "C" int bc_poseidon_build_merkle_tree_nodes(const uint64_t *values, uint64_t values_len, uint64_t *results,
                                                    uint64_t results_len, uint32_t layers_count, void *stream) {
-  using namespace bc;
-  // src/poseidon.rs:262-286: each layer halves; its nodes fill the first half
-  // of what remains of `results`
-  if (layers_count == 0) return BC_SUCCESS;
-  if (values_len % RATE != 0 || values_len != results_len) return BC_ERROR_INVALID_VALUE;
-  const uint64_t leaves = values_len / RATE;
-  if (leaves == 0 || (leaves & (leaves - 1)) != 0) return BC_ERROR_INVALID_VALUE;
-  if (layers_count > 64 || (layers_count > 1 && (values_len >> (layers_count - 1)) < RATE))
-    return BC_ERROR_INVALID_VALUE;
-  const int rc = require_table();
-  if (rc != BC_SUCCESS) return rc;
-  const uint64_t *in = values;
-  uint64_t len = values_len;
-  uint64_t *out = results;
-  for (uint32_t l = 0; l < layers_count; ++l) {
-    const uint64_t half = len / 2;
-    const int e = bc_poseidon_nodes(in, len, out, half, stream);
-    if (e != BC_SUCCESS) return e;
-    in = out;
-    out += half;
-    len = half;
-  }
-  return BC_SUCCESS;
+  return bc::tree_nodes_impl<0>(values, values_len, results, results_len, layers_count, stream);
 }
 
 extern "C" int bc_poseidon_build_merkle_tree(const uint64_t *values, uint64_t values_len, uint64_t *results,
                                              uint64_t results_len, uint32_t log_rows_per_hash,
                                              uint32_t layers_count, void *stream) {
-  using namespace bc;
-  // src/poseidon.rs:311-328: the leaf digests fill the first half of
-  // `results`, the node layers the rest
-  if (layers_count == 0 || results_len % (2 * CAP) != 0) return BC_ERROR_INVALID_VALUE;
-  const uint64_t leaves = results_len / (2 * CAP);
-  if (leaves == 0 || layers_count - 1 >= 64 || (1ull << (layers_count - 1)) > leaves ||
-      values_len % leaves != 0)
-    return BC_ERROR_INVALID_VALUE;
-  const uint64_t half = results_len / 2;
-  int e = bc_poseidon_leaves(values, values_len, results, half, log_rows_per_hash, false, false, stream);
-  if (e != BC_SUCCESS) return e;
-  return bc_poseidon_build_merkle_tree_nodes(results, half, results + half, half, layers_count - 1, stream);
+  return bc::tree_impl<0>(values, values_len, results, results_len, log_rows_per_hash, layers_count, stream);
+}
+
+extern "C" int bc_poseidon2_leaves(const uint64_t *values, uint64_t values_len, uint64_t *results,
+                                   uint64_t results_len, uint32_t log_rows_per_hash, bool load_intermediate,
+                                   bool store_intermediate, void *stream) {
+  return bc::leaves_impl<1>(values, values_len, results, results_len, log_rows_per_hash, load_intermediate,
+                            store_intermediate, stream);
+}
+
+extern "C" int bc_poseidon2_nodes(const uint64_t *values, uint64_t values_len, uint64_t *results,
+                                  uint64_t results_len, void *stream) {
+  return bc::nodes_impl<1>(values, values_len, results, results_len, stream);
+}
+
+extern "C" int bc_poseidon2_build_merkle_tree_nodes(const uint64_t *values, uint64_t values_len, uint64_t *results,
+                                                    uint64_t results_len, uint32_t layers_count, void *stream) {
+  return bc::tree_nodes_impl<1>(values, values_len, results, results_len, layers_count, stream);
+}
+
+extern "C" int bc_poseidon2_build_merkle_tree(const uint64_t *values, uint64_t values_len, uint64_t *results,
+                                              uint64_t results_len, uint32_t log_rows_per_hash,
+                                              uint32_t layers_count, void *stream) {
+  return bc::tree_impl<1>(values, values_len, results, results_len, log_rows_per_hash, layers_count, stream);
 }

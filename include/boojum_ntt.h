@@ -229,7 +229,7 @@ int bc_batch_eval(uint32_t kind, const uint64_t *ys, uint64_t stride, uint32_t n
                   uint64_t *temp_partial, size_t temp_partial_elems, void *temp_final, size_t temp_final_bytes,
                   uint64_t *evals, void *stream);
 
-/* ---- Poseidon Merkle trees (reference: src/poseidon.rs:129-328) ----
+/* ---- Poseidon / Poseidon2 Merkle trees (reference: src/poseidon.rs:129-328) ----
  * Poseidon over Goldilocks: width 12 (rate 8, capacity 4), 8 full and 22
  * partial rounds, S-box x^7, MDS = circulant of 2^{0,0,1,0,3,5,1,8,12,3,16,10}
  * (native/poseidon_constants_template.cuh:26).  Digests are 4 elements,
@@ -245,10 +245,12 @@ int bc_batch_eval(uint32_t kind, const uint64_t *ys, uint64_t stride, uint32_t n
  * device returns BC_ERROR_NOT_INITIALIZED until this has been called there. */
 int bc_poseidon_set_round_constants(const uint64_t *round_constants);
 
-/* Host only (no device call): the fused table derived from 360 constants, as
- * out_words >= 746 u64: textbook constants of rounds 0-3 and 27-29 (84), the
- * fused constant layer (12), the dense matrix (144, row-major), the fused
- * S-box constants (22), VS (22 x 11) and W_HATS (22 x 11). */
+/* Host only (no device call): the per-device table derived from 360
+ * constants, as out_words >= 864 u64.  Poseidon's fused form: textbook
+ * constants of rounds 0-3 and 27-29 (84), the fused constant layer (12), the
+ * dense matrix (144, row-major), the fused S-box constants (22), VS (22 x 11)
+ * and W_HATS (22 x 11); then Poseidon2's constants: full rounds 0-3 and
+ * 26-29 (96) and element 0 of each partial round (22). */
 int bc_poseidon_fused_table(const uint64_t *round_constants, uint64_t *out, uint64_t out_words);
 
 /* launch_leaves_kernel (src/poseidon.rs:129-170): count = results_len / 4
@@ -279,6 +281,21 @@ int bc_poseidon_build_merkle_tree_nodes(const uint64_t *values, uint64_t values_
 int bc_poseidon_build_merkle_tree(const uint64_t *values, uint64_t values_len, uint64_t *results,
                                   uint64_t results_len, uint32_t log_rows_per_hash, uint32_t layers_count,
                                   void *stream);
+
+/* Poseidon2 (native/poseidon2_single_thread.cu, the hash the reference's
+ * tests exercise): the same entry points, layouts and argument checks as the
+ * Poseidon ones above, with the Poseidon2 permutation (external matrix
+ * circ(2 M4, M4, M4), internal matrix J + diag(2^{4,14,11,8,0,5,2,9,13,6,3,12}),
+ * eprint 2023/323) over the same round constants. */
+int bc_poseidon2_leaves(const uint64_t *values, uint64_t values_len, uint64_t *results, uint64_t results_len,
+                        uint32_t log_rows_per_hash, bool load_intermediate, bool store_intermediate, void *stream);
+int bc_poseidon2_nodes(const uint64_t *values, uint64_t values_len, uint64_t *results, uint64_t results_len,
+                       void *stream);
+int bc_poseidon2_build_merkle_tree_nodes(const uint64_t *values, uint64_t values_len, uint64_t *results,
+                                         uint64_t results_len, uint32_t layers_count, void *stream);
+int bc_poseidon2_build_merkle_tree(const uint64_t *values, uint64_t values_len, uint64_t *results,
+                                   uint64_t results_len, uint32_t log_rows_per_hash, uint32_t layers_count,
+                                   void *stream);
 
 /* Seeded inputs on the device: column c (c = 0..num_cols-1) at
  * out + c * stride gets 2^log_n u64 draws of the splitmix64 stream seeded with

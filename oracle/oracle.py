@@ -68,6 +68,10 @@ def lib():
             "pos_leaves": (None, [_u64p, _u64p, u32, u32, u32, i32, i32, _u64p, _u64p]),
             "pos_nodes": (None, [_u64p, _u64p, u32, _u64p, _u64p]),
             "pos_grain_constants": (None, [_u64p]),
+            "pos2_matrices": (None, [_u64p, _u64p]),
+            "pos2_permute": (None, [_u64p, _u64p]),
+            "pos2_leaves": (None, [_u64p, _u64p, u32, u32, u32, i32, i32, _u64p]),
+            "pos2_nodes": (None, [_u64p, _u64p, u32, _u64p]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -317,4 +321,35 @@ def pos_nodes(values, count, rc, fused_words):
     out = np.zeros(4 * count, dtype=np.uint64)
     lib().pos_nodes(_ptr(values), _ptr(out), count, _ptr(np.ascontiguousarray(rc, dtype=np.uint64)),
                     _ptr(fused_words))
+    return out
+
+
+def pos2_matrices():
+    m_e = np.zeros((12, 12), dtype=np.uint64)
+    m_i = np.zeros((12, 12), dtype=np.uint64)
+    lib().pos2_matrices(_ptr(m_e), _ptr(m_i))
+    return m_e, m_i
+
+
+def pos2_permute(state, rc):
+    s = np.ascontiguousarray(state, dtype=np.uint64).copy()
+    lib().pos2_permute(_ptr(s), _ptr(np.ascontiguousarray(rc, dtype=np.uint64)))
+    return s
+
+
+def pos2_leaves(values, rows, cols, count, rc, results=None, load_intermediate=False, store_intermediate=False):
+    values = np.ascontiguousarray(values, dtype=np.uint64)
+    assert values.size >= rows * cols * count
+    if results is None:
+        results = np.zeros(4 * count, dtype=np.uint64)
+    lib().pos2_leaves(_ptr(values), _ptr(results), rows, cols, count, int(load_intermediate),
+                      int(store_intermediate), _ptr(np.ascontiguousarray(rc, dtype=np.uint64)))
+    return results
+
+
+def pos2_nodes(values, count, rc):
+    values = np.ascontiguousarray(values, dtype=np.uint64)
+    assert values.size >= 8 * count
+    out = np.zeros(4 * count, dtype=np.uint64)
+    lib().pos2_nodes(_ptr(values), _ptr(out), count, _ptr(np.ascontiguousarray(rc, dtype=np.uint64)))
     return out

@@ -24,11 +24,14 @@
  * constants (ALL_ROUND_CONSTANTS of boojum::implementations::
  * poseidon_goldilocks_params, crate boojum = "=0.2.2") at build time
  * (build/poseidon_constants.rs:66-75) and derives the fused form from them.
- * Those constants are not in /root/reference, so this oracle takes the 360
+ * Those constants are not in the reference tree, so this oracle takes the 360
  * constants as an argument and derives the fused form itself
  * (pos_derive): moving the partial rounds' constants backwards through
  * M^-1 and factoring each partial round's MDS as B_k A_k, B_k sparse and A_k
  * = diag(1, N^), where N starts at M and becomes A_k M for the round before.
+ *
+ * Poseidon2 (pos2_*, at the end of this file) restates the reference's
+ * second hash with the same constants, in textbook matrix form.
  *
  * Parity pinning (tests/test_oracle_poseidon.py):
  *   - the derived fused matrices (dense, VS, W_HATS), which depend only on the
@@ -36,7 +39,10 @@
  *     (checksums and spot values in tests/golden/poseidon_fused.json);
  *   - the fused permutation equals the textbook one (constants before each
  *     round's S-box, full / partial S-box layer, MDS) for any constants;
- *   - leaf and node hashing against an independent Python sponge.
+ *   - leaf and node hashing against an independent Python sponge;
+ *   - Poseidon2: M4 entries equal the paper's, mu equals the reference's
+ *     LOG_MU_MINUS_ONE (fixture), both matrices invertible mod p, the
+ *     permutation against a Python restatement built tile by tile.
  * Parity unpinned: the constant values themselves (boojum's), hence every
  * digest computed with real boojum constants.  pos_grain_constants
  * generates constants by the Poseidon reference parameter procedure (Grain
@@ -326,4 +332,73 @@ void pos_grain_constants(uint64_t *out) {
 #undef GRAIN_STEP
 #undef GRAIN_BIT
   (void)tmp;
+}
+
+/* ---- Poseidon2 (native/poseidon2_single_thread.cu:7-24,
+ * native/poseidon2_single_thread.cuh:12-66) ----
+ * Same width, rounds, S-box and round constants (the reference's
+ * poseidon_common::ALL_ROUND_CONSTANTS serves both hashes).  Textbook form
+ * with explicit matrices:
+ *   - external M_E = circ(2 M4, M4, M4) over the three 4-tiles, M4 =
+ *     [[5,7,1,3],[4,6,1,1],[1,3,5,7],[1,1,4,6]] (eprint 2023/323, App. B);
+ *   - internal M_I = J + diag(2^mu_i), mu = LOG_MU_MINUS_ONE
+ *     (poseidon_constants_template.cuh:125), so M_I[i][i] = 2^mu_i + 1;
+ *   - M_E first, then full rounds (constants on all 12, S-box on all, M_E)
+ *     0-3 and 26-29, partial rounds 4-25 (constant and S-box on element 0,
+ *     M_I). */
+static const uint64_t P2_M4[4][4] = {{5, 7, 1, 3}, {4, 6, 1, 1}, {1, 3, 5, 7}, {1, 1, 4, 6}};
+static const unsigned P2_MU[PO_T] = {4, 14, 11, 8, 0, 5, 2, 9, 13, 6, 3, 12};
+
+void pos2_matrices(uint64_t *m_e, uint64_t *m_i) {
+  for (int r = 0; r < PO_T; ++r)
+    for (int c = 0; c < PO_T; ++c) {
+      const uint64_t m4 = P2_M4[r % 4][c % 4];
+      m_e[r * PO_T + c] = (r / 4 == c / 4) ? 2 * m4 : m4;
+      m_i[r * PO_T + c] = 1 + (r == c ? (1ull << P2_MU[r]) : 0);
+    }
+}
+
+void pos2_permute(uint64_t *s, const uint64_t *rc) {
+  uint64_t m_e[PO_T * PO_T], m_i[PO_T * PO_T];
+  pos2_matrices(m_e, m_i);
+  for (int i = 0; i < PO_T; ++i) s[i] %= PO_P;
+  mat_vec(m_e, PO_T, s, s);
+  for (int round = 0; round < PO_ROUNDS; ++round) {
+    const int full = round < PO_HALF_FULL || round >= PO_HALF_FULL + PO_PARTIAL;
+    for (int i = 0; i < PO_T; ++i) {
+      if (full || i == 0) s[i] = po_sbox(po_add(s[i], rc[round * PO_T + i]));
+    }
+    mat_vec(full ? m_e : m_i, PO_T, s, s);
+  }
+}
+
+/* Leaves and nodes as pos_leaves / pos_nodes, with the Poseidon2
+ * permutation. */
+void pos2_leaves(const uint64_t *values, uint64_t *results, uint32_t rows, uint32_t cols, uint32_t count,
+                 int load_intermediate, int store_intermediate, const uint64_t *rc) {
+  for (uint32_t g = 0; g < count; ++g) {
+    uint64_t s[PO_T] = {0};
+    if (load_intermediate)
+      for (int i = 0; i < PO_CAP; ++i) s[PO_RATE + i] = results[g + (uint64_t)i * count];
+    const uint64_t total = (uint64_t)rows * cols;
+    for (uint64_t off = 0; off < total;) {
+      for (int i = 0; i < PO_RATE; ++i, ++off) {
+        const uint64_t r = off % rows, c = off / rows;
+        s[i] = c < cols ? values[(uint64_t)g * rows + r + c * (uint64_t)rows * count] : 0;
+      }
+      pos2_permute(s, rc);
+    }
+    for (int i = 0; i < PO_CAP; ++i)
+      results[g + (uint64_t)i * count] = store_intermediate ? s[PO_RATE + i] : s[i];
+  }
+}
+
+void pos2_nodes(const uint64_t *values, uint64_t *results, uint32_t count, const uint64_t *rc) {
+  for (uint32_t g = 0; g < count; ++g) {
+    uint64_t s[PO_T] = {0};
+    for (int i = 0; i < PO_CAP; ++i)
+      for (int j = 0; j < 2; ++j) s[j * PO_CAP + i] = values[2 * ((uint64_t)g + (uint64_t)i * count) + j];
+    pos2_permute(s, rc);
+    for (int i = 0; i < PO_CAP; ++i) results[g + (uint64_t)i * count] = s[i];
+  }
 }

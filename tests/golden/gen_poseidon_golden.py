@@ -51,6 +51,8 @@ def main():
         out["tables"][name] = {"count": len(v), "sha256": digest(v), "first": hex(v[0]), "last": hex(v[-1])}
     mds = re.search(r"MDS_MATRIX_EXPS\[STATE_WIDTH\] = \{([0-9,]+)\};", text).group(1)
     out["mds_matrix_exps"] = [int(x) for x in mds.strip(",").split(",")]
+    mu = re.search(r"LOG_MU_MINUS_ONE\[STATE_WIDTH\] = \{([0-9, ]+)\};", text).group(1)
+    out["poseidon2_log_mu_minus_one"] = [int(x) for x in mu.replace(" ", "").strip(",").split(",")]
     with open(os.path.join(HERE, "poseidon_fused.json"), "w") as f:
         json.dump(out, f, indent=1)
         f.write("\n")

@@ -174,3 +174,94 @@ def test_nodes_match_python(grain):
         right = [int(vals[i * 2 * count + 2 * g + 1]) for i in range(4)]
         want = py_permute(left + right + [0] * 4, rc)[:4]
         assert [int(got[g + i * count]) for i in range(4)] == want
+
+
+# ---- Poseidon2 ----------------------------------------------------------------
+
+M4_PAPER = [[5, 7, 1, 3], [4, 6, 1, 1], [1, 3, 5, 7], [1, 1, 4, 6]]  # eprint 2023/323, Appendix B
+MU = GOLDEN["poseidon2_log_mu_minus_one"]
+
+
+def py2_permute(state, rc):
+    """Poseidon2 built tile by tile: M_E = M4 on each 4-tile, then every tile
+    plus the sum of all tiles; M_I = the sum of all elements plus 2^mu_i x_i."""
+    def m_e(s):
+        t = [sum(M4_PAPER[r][c] * s[4 * k + c] for c in range(4)) % P for k in range(3) for r in range(4)]
+        tot = [sum(t[4 * k + j] for k in range(3)) for j in range(4)]
+        return [(t[i] + tot[i % 4]) % P for i in range(12)]
+
+    def m_i(s):
+        tot = sum(s)
+        return [(tot + (x << MU[i])) % P for i, x in enumerate(s)]
+
+    s = m_e([int(x) % P for x in state])
+    for r in range(30):
+        full = r < 4 or r >= 26
+        s = [pow((x + int(rc[r * 12 + i])) % P, 7, P) if (full or i == 0) else x for i, x in enumerate(s)]
+        s = m_e(s) if full else m_i(s)
+    return s
+
+
+def det_mod_p(m):
+    a = [[int(x) % P for x in row] for row in m]
+    n, d = len(a), 1
+    for c in range(n):
+        piv = next((r for r in range(c, n) if a[r][c]), None)
+        if piv is None:
+            return 0
+        if piv != c:
+            a[c], a[piv] = a[piv], a[c]
+            d = -d
+        d = d * a[c][c] % P
+        inv = pow(a[c][c], P - 2, P)
+        for r in range(c + 1, n):
+            f = a[r][c] * inv % P
+            a[r] = [(x - f * y) % P for x, y in zip(a[r], a[c])]
+    return d % P
+
+
+def test_poseidon2_matrices():
+    m_e, m_i = oracle.pos2_matrices()
+    for k in range(3):
+        for l in range(3):
+            block = m_e[4 * k:4 * k + 4, 4 * l:4 * l + 4].astype(object)
+            assert (block == np.array(M4_PAPER, dtype=object) * (2 if k == l else 1)).all()
+    assert [int(m_i[i, i]) - 1 for i in range(12)] == [1 << m for m in MU]
+    assert all(int(m_i[i, j]) == 1 for i in range(12) for j in range(12) if i != j)
+    assert det_mod_p(m_e) != 0 and det_mod_p(m_i) != 0
+
+
+@pytest.mark.parametrize("const_seed", [None, 21])
+def test_poseidon2_permutation(grain, const_seed):
+    rc = grain[0] if const_seed is None else np.random.default_rng(const_seed).integers(0, P, 360, dtype=np.uint64)
+    rng = np.random.default_rng(8)
+    for x in [np.zeros(12, dtype=np.uint64), np.full(12, P - 1, dtype=np.uint64)] + \
+             [rng.integers(0, 2**64, 12, dtype=np.uint64) for _ in range(4)]:
+        assert [int(v) for v in oracle.pos2_permute(x, rc)] == py2_permute(x, rc)
+
+
+def test_poseidon2_differs_from_poseidon(grain):
+    x = np.arange(12, dtype=np.uint64)
+    assert not np.array_equal(oracle.pos2_permute(x, grain[0]), oracle.pos_permute_fused(x, grain[0], grain[1]["words"]))
+
+
+@pytest.mark.parametrize("rows,cols", [(1, 1), (1, 13), (2, 4), (4, 3)])
+def test_poseidon2_leaves_and_nodes(grain, rows, cols):
+    rc = grain[0]
+    count = 3
+    vals = np.random.default_rng(rows * 10 + cols).integers(0, 2**64, rows * cols * count, dtype=np.uint64)
+    got = oracle.pos2_leaves(vals, rows, cols, count, rc)
+    for g in range(count):
+        s = [0] * 12
+        flat = [int(vals[g * rows + r + c * rows * count]) for c in range(cols) for r in range(rows)]
+        for k in range(0, len(flat), 8):
+            chunk = flat[k:k + 8]
+            s[:8] = chunk + [0] * (8 - len(chunk))
+            s = py2_permute(s, rc)
+        assert [int(got[g + i * count]) for i in range(4)] == s[:4]
+    nvals = np.random.default_rng(cols).integers(0, P, 8 * count, dtype=np.uint64)
+    nodes = oracle.pos2_nodes(nvals, count, rc)
+    for g in range(count):
+        left = [int(nvals[i * 2 * count + 2 * g]) for i in range(4)]
+        right = [int(nvals[i * 2 * count + 2 * g + 1]) for i in range(4)]
+        assert [int(nodes[g + i * count]) for i in range(4)] == py2_permute(left + right + [0] * 4, rc)[:4]

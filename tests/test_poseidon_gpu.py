@@ -1,4 +1,5 @@
-"""GPU parity of Poseidon Merkle-tree hashing (src/poseidon.rs:129-328),
+"""GPU parity of Poseidon and Poseidon2 Merkle-tree hashing
+(src/poseidon.rs:129-328),
 shaped like the reference's own tests (src/poseidon.rs:495-700: leaves with
 and without checkpoints, nodes, whole trees), bit-exact against the oracle
 (oracle/poseidon_oracle.c) with the same 360 round constants.
@@ -36,13 +37,25 @@ def consts(gpu_ctx):
     return rc, oracle.pos_derive(rc)["words"]
 
 
-def oracle_leaves(vals, rows, cols, count, consts, **kw):
+HASHES = ["poseidon", "poseidon2"]
+
+
+def oracle_leaves(vals, rows, cols, count, consts, p="poseidon", **kw):
+    if p == "poseidon2":
+        return oracle.pos2_leaves(vals, rows, cols, count, consts[0], **kw)
     return oracle.pos_leaves(vals, rows, cols, count, consts[0], consts[1], **kw)
 
 
+def oracle_nodes(vals, count, consts, p="poseidon"):
+    if p == "poseidon2":
+        return oracle.pos2_nodes(vals, count, consts[0])
+    return oracle.pos_nodes(vals, count, consts[0], consts[1])
+
+
+@pytest.mark.parametrize("p", HASHES)
 @pytest.mark.parametrize("log_rows", [0, 1, 2, 3])
 @pytest.mark.parametrize("cols", [1, 3, 8, 13])
-def test_leaves_match_oracle(consts, log_rows, cols):
+def test_leaves_match_oracle(consts, p, log_rows, cols):
     """src/poseidon.rs:495-550 (test_leaves): random values, several row
     groupings and column counts (13 pads the last chunk)."""
     import torch
@@ -50,12 +63,13 @@ def test_leaves_match_oracle(consts, log_rows, cols):
     rows, count = 1 << log_rows, 300
     vals = np.random.default_rng(log_rows * 31 + cols).integers(0, 2**64, rows * cols * count, dtype=np.uint64)
     res = torch.zeros(4 * count, dtype=torch.int64, device="cuda")
-    poseidon.launch_single_thread_leaves_kernel(to_dev(vals), res, log_rows)
-    assert np.array_equal(to_host(res), oracle_leaves(vals, rows, cols, count, consts))
+    poseidon.launch_single_thread_leaves_kernel(to_dev(vals), res, log_rows, p=p)
+    assert np.array_equal(to_host(res), oracle_leaves(vals, rows, cols, count, consts, p))
 
 
+@pytest.mark.parametrize("p", HASHES)
 @pytest.mark.parametrize("rows,c1,c2", [(1, 8, 5), (2, 4, 3), (8, 1, 2), (1, 16, 8)])
-def test_leaves_checkpointed(consts, rows, c1, c2):
+def test_leaves_checkpointed(consts, p, rows, c1, c2):
     """src/poseidon.rs:568-581 (checkpointed leaves): the columns in two calls,
     the first storing the capacity and the second loading it, equal one call."""
     import torch
@@ -65,26 +79,28 @@ def test_leaves_checkpointed(consts, rows, c1, c2):
     vals = np.random.default_rng(rows * 7 + c1).integers(0, P, rows * (c1 + c2) * count, dtype=np.uint64)
     split = rows * c1 * count
     res = torch.zeros(4 * count, dtype=torch.int64, device="cuda")
-    poseidon.launch_single_thread_leaves_kernel(to_dev(vals[:split]), res, log_rows, store_intermediate=True)
+    poseidon.launch_single_thread_leaves_kernel(to_dev(vals[:split]), res, log_rows, store_intermediate=True, p=p)
     mid = to_host(res).copy()
-    assert np.array_equal(mid, oracle_leaves(vals[:split], rows, c1, count, consts, store_intermediate=True))
-    poseidon.launch_single_thread_leaves_kernel(to_dev(vals[split:]), res, log_rows, load_intermediate=True)
-    assert np.array_equal(to_host(res), oracle_leaves(vals, rows, c1 + c2, count, consts))
+    assert np.array_equal(mid, oracle_leaves(vals[:split], rows, c1, count, consts, p, store_intermediate=True))
+    poseidon.launch_single_thread_leaves_kernel(to_dev(vals[split:]), res, log_rows, load_intermediate=True, p=p)
+    assert np.array_equal(to_host(res), oracle_leaves(vals, rows, c1 + c2, count, consts, p))
 
 
+@pytest.mark.parametrize("p", HASHES)
 @pytest.mark.parametrize("count", [1, 2, 255, 256, 1000])
-def test_nodes_match_oracle(consts, count):
+def test_nodes_match_oracle(consts, p, count):
     """src/poseidon.rs:583-626 (test_nodes)."""
     import torch
     from boojum_cuda_amd import poseidon
     vals = np.random.default_rng(count).integers(0, 2**64, 8 * count, dtype=np.uint64)
     res = torch.zeros(4 * count, dtype=torch.int64, device="cuda")
-    poseidon.launch_single_thread_nodes_kernel(to_dev(vals), res)
-    assert np.array_equal(to_host(res), oracle.pos_nodes(vals, count, consts[0], consts[1]))
+    poseidon.launch_single_thread_nodes_kernel(to_dev(vals), res, p=p)
+    assert np.array_equal(to_host(res), oracle_nodes(vals, count, consts, p))
 
 
+@pytest.mark.parametrize("p", HASHES)
 @pytest.mark.parametrize("log_leaves,log_rows,cols", [(0, 0, 3), (4, 0, 8), (10, 1, 5), (12, 0, 16)])
-def test_merkle_tree_matches_oracle(consts, log_leaves, log_rows, cols):
+def test_merkle_tree_matches_oracle(consts, p, log_leaves, log_rows, cols):
     """src/poseidon.rs:628-694 (merkle_tree): every layer of the tree."""
     import torch
     from boojum_cuda_amd import poseidon
@@ -92,19 +108,20 @@ def test_merkle_tree_matches_oracle(consts, log_leaves, log_rows, cols):
     layers = log_leaves + 1
     vals = np.random.default_rng(log_leaves).integers(0, P, leaves * rows * cols, dtype=np.uint64)
     res = torch.zeros(8 * leaves, dtype=torch.int64, device="cuda")
-    poseidon.build_merkle_tree(to_dev(vals), res, log_rows, layers_count=layers)
+    poseidon.build_merkle_tree(to_dev(vals), res, log_rows, layers_count=layers, p=p)
     got = to_host(res)
-    layer = oracle_leaves(vals, rows, cols, leaves, consts)
+    layer = oracle_leaves(vals, rows, cols, leaves, consts, p)
     assert np.array_equal(got[:4 * leaves], layer)
     off, n = 4 * leaves, leaves
     for _ in range(layers - 1):
         n //= 2
-        layer = oracle.pos_nodes(layer, n, consts[0], consts[1])
+        layer = oracle_nodes(layer, n, consts, p)
         assert np.array_equal(got[off:off + 4 * n], layer)
         off += 4 * n
 
 
-def test_leaves_large_sampled(consts):
+@pytest.mark.parametrize("p", HASHES)
+def test_leaves_large_sampled(consts, p):
     """2^20 leaves x 8 columns (one 8-element chunk each), 64 sampled leaves
     against the oracle, plus a second call giving the same digests."""
     import torch
@@ -114,16 +131,16 @@ def test_leaves_large_sampled(consts):
     g.manual_seed(20)
     vals = torch.randint(-2**63, 2**63 - 1, (cols * count,), dtype=torch.int64, device="cuda", generator=g)
     res = torch.zeros(4 * count, dtype=torch.int64, device="cuda")
-    poseidon.launch_single_thread_leaves_kernel(vals, res, 0)
+    poseidon.launch_single_thread_leaves_kernel(vals, res, 0, p=p)
     got = to_host(res)
     host = to_host(vals)
     pick = np.random.default_rng(1).choice(count, 64, replace=False)
     sub = np.concatenate([host[c * count + pick] for c in range(cols)])  # the picked leaves as a 64-leaf matrix
-    want = oracle_leaves(sub, 1, cols, 64, consts)
+    want = oracle_leaves(sub, 1, cols, 64, consts, p)
     for i in range(4):
         assert np.array_equal(got[i * count + pick], want[i * 64:(i + 1) * 64])
     res2 = torch.zeros_like(res)
-    poseidon.launch_single_thread_leaves_kernel(vals, res2, 0)
+    poseidon.launch_single_thread_leaves_kernel(vals, res2, 0, p=p)
     assert torch.equal(res, res2)
 
 
@@ -131,8 +148,9 @@ def test_empty_is_noop(consts):
     import torch
     from boojum_cuda_amd import poseidon
     e = torch.zeros(0, dtype=torch.int64, device="cuda")
-    poseidon.launch_single_thread_leaves_kernel(e, e, 0)
-    poseidon.launch_single_thread_nodes_kernel(e, e)
+    for p in HASHES:
+        poseidon.launch_single_thread_leaves_kernel(e, e, 0, p=p)
+        poseidon.launch_single_thread_nodes_kernel(e, e, p=p)
 
 
 def test_bad_arguments_raise(consts):
@@ -143,3 +161,17 @@ def test_bad_arguments_raise(consts):
         poseidon.launch_single_thread_leaves_kernel(v, torch.zeros(6, dtype=torch.int64, device="cuda"), 0)
     with pytest.raises(AssertionError):
         poseidon.launch_single_thread_nodes_kernel(v, torch.zeros(8, dtype=torch.int64, device="cuda"))
+    with pytest.raises(AssertionError):
+        poseidon.launch_single_thread_leaves_kernel(v, torch.zeros(6, dtype=torch.int64, device="cuda"), 0,
+                                                    p="poseidon2")
+
+
+def test_hashes_differ(consts):
+    import torch
+    from boojum_cuda_amd import poseidon
+    vals = to_dev(np.arange(8 * 4, dtype=np.uint64))
+    a = torch.zeros(16, dtype=torch.int64, device="cuda")
+    b = torch.zeros(16, dtype=torch.int64, device="cuda")
+    poseidon.launch_single_thread_leaves_kernel(vals, a, 0, p="poseidon")
+    poseidon.launch_single_thread_leaves_kernel(vals, b, 0, p="poseidon2")
+    assert not torch.equal(a, b)

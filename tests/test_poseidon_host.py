@@ -1,4 +1,4 @@
-"""The library's own host-side Poseidon derivation (bc_poseidon_fused_table,
+"""The library's own host-side Poseidon / Poseidon2 table (bc_poseidon_fused_table,
 no device call) against the oracle and against the reference's fused tables
 (native/poseidon_constants_template.cuh:43-124, through the checksums in
 tests/golden/poseidon_fused.json), plus the argument checks of the hashing
@@ -39,6 +39,10 @@ def test_library_derivation_equals_oracle(seed):
     canon = rc % np.uint64(oracle.P) if seed is not None else rc
     assert np.array_equal(got["rc_full"][:4], canon[:48].reshape(4, 12))
     assert np.array_equal(got["rc_full"][4:], canon[27 * 12:].reshape(3, 12))
+    # Poseidon2 reads the textbook constants directly
+    assert np.array_equal(got["p2_full"][:4], canon[:48].reshape(4, 12))
+    assert np.array_equal(got["p2_full"][4:], canon[26 * 12:].reshape(4, 12))
+    assert np.array_equal(got["p2_partial"], canon[4 * 12:26 * 12:12])
 
 
 @pytest.mark.parametrize("key,name", [
