@@ -273,6 +273,141 @@ __global__ __launch_bounds__(256) void poseidon_nodes_kernel(const uint64_t *__r
   for (int i = 0; i < CAP; ++i) results[g + (uint64_t)i * count] = s[i];
 }
 
+// ---- cooperative node hashing: one state over 16 lanes ------------------------
+// A small layer is one dependent permutation chain per lane, about 28k
+// instructions: ~91 us however few nodes the layer has.  Here lane j of a
+// 16-lane group owns state element j (lanes 12-15 compute values nobody
+// reads), so each lane runs one S-box and one matrix row per round; matrix
+// rows gather the other elements with __shfl inside the group, and the
+// constants are staged in LDS.  Used for layers of at most COOP_MAX_NODES
+// nodes (the reference likewise switches to cooperative kernels for small
+// layers, src/poseidon.rs:273-284).
+constexpr uint32_t COOP_LANES = 16, COOP_MAX_NODES = 1u << 14;
+
+__device__ __forceinline__ uint64_t grp(uint64_t v, uint32_t lane) { return __shfl(v, (int)lane, (int)COOP_LANES); }
+
+// sum over the 16 lanes of the group (lanes 12-15 must contribute 0)
+__device__ __forceinline__ uint64_t grp_sum(uint64_t v) {
+#pragma unroll
+  for (uint32_t m = 8; m >= 1; m >>= 1) v = gl::add(v, __shfl_xor(v, (int)m, (int)COOP_LANES));
+  return v;
+}
+
+__device__ __forceinline__ uint64_t coop_mds_row(uint64_t x, uint32_t j) {
+  uint64_t lo = 0;
+  uint32_t hi = 0;
+#pragma unroll
+  for (int i = 0; i < PT; ++i) {
+    const uint64_t v = grp(x, (j + (uint32_t)i) % PT);
+    const unsigned e = MDS_EXPS[i];
+    const uint64_t t = e ? v << e : v;
+    lo += t;
+    hi += (e ? (uint32_t)(v >> (64 - e)) : 0u) + (lo < t ? 1u : 0u);
+  }
+  return gl::reduce_limbs((uint32_t)lo, (uint32_t)(lo >> 32), hi, 0u);
+}
+
+__device__ __forceinline__ void coop_permute(uint64_t &x, uint32_t j, const PosTable &t) {
+  const uint32_t jj = j < PT ? j : PT - 1;  // table row of this lane (lanes 12-15 reuse row 11)
+  const uint32_t jm = jj ? jj - 1 : 0;      // index into the 11-entry sparse vectors
+#pragma unroll 1
+  for (int r = 0; r < HALF_FULL - 1; ++r) x = coop_mds_row(sbox(gl::add(x, t.rc_full[r][jj])), j);
+  {
+    const uint64_t v = gl::add(sbox(gl::add(x, t.rc_full[HALF_FULL - 1][jj])), t.rcf[jj]);
+    gl::Acc acc;
+#pragma unroll
+    for (int c = 0; c < PT; ++c) gl::mac(acc, grp(v, (uint32_t)c), t.dense[jj][c]);
+    x = gl::acc_reduce(acc);
+  }
+#pragma unroll 1
+  for (int k = 0; k < PARTIAL; ++k) {
+    const uint64_t s0 = grp(gl::add(sbox(x), t.sbox_c[k]), 0);
+    const uint64_t term = j == 0 ? s0 : (j < PT ? gl::mul(x, t.vs[k][jm]) : 0ull);
+    const uint64_t y0 = grp_sum(term);
+    x = j == 0 ? y0 : gl::add(gl::mul(s0, t.w_hats[k][jm]), x);
+  }
+  x = coop_mds_row(sbox(x), j);
+#pragma unroll 1
+  for (int r = 0; r < 3; ++r) x = coop_mds_row(sbox(gl::add(x, t.rc_full[HALF_FULL + r][jj])), j);
+}
+
+// sum_c m_c v_c for small m_c (M4 entries, at most 7): 67-bit products
+// summed in 96 bits, one reduction
+__device__ __forceinline__ uint64_t small_dot4(const uint64_t *v, const uint32_t *m) {
+  uint64_t lo = 0;
+  uint32_t hi = 0;
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const uint64_t p = v[c] * m[c];
+    const uint32_t ph = (uint32_t)__umul64hi(v[c], (uint64_t)m[c]);
+    lo += p;
+    hi += ph + (lo < p ? 1u : 0u);
+  }
+  return gl::reduce_limbs((uint32_t)lo, (uint32_t)(lo >> 32), hi, 0u);
+}
+
+__device__ __forceinline__ uint64_t coop_m_ext(uint64_t x, uint32_t j, const uint32_t *m4row) {
+  const uint32_t tile = j & ~3u, q = j & 3u;
+  uint64_t v[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) v[c] = grp(x, tile + (uint32_t)c);
+  const uint64_t y = small_dot4(v, m4row);
+  const uint64_t tot = gl::add(gl::add(grp(y, q), grp(y, 4 + q)), grp(y, 8 + q));
+  return gl::add(y, tot);
+}
+
+__device__ __forceinline__ uint64_t coop_m_int(uint64_t x, uint32_t j) {
+  const uint64_t sum = grp_sum(j < PT ? x : 0ull);
+  const unsigned e = P2_MU[j < PT ? j : 0];
+  uint64_t lo = e ? x << e : x;
+  uint32_t hi = e ? (uint32_t)(x >> (64 - e)) : 0u;
+  lo += sum;
+  hi += lo < sum ? 1u : 0u;
+  return gl::reduce_limbs((uint32_t)lo, (uint32_t)(lo >> 32), hi, 0u);
+}
+
+__device__ __forceinline__ void coop_permute2(uint64_t &x, uint32_t j, const PosTable &t) {
+  constexpr uint32_t M4[4][4] = {{5, 7, 1, 3}, {4, 6, 1, 1}, {1, 3, 5, 7}, {1, 1, 4, 6}};
+  uint32_t m4row[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const uint32_t q = j & 3u;
+    m4row[c] = q == 0 ? M4[0][c] : (q == 1 ? M4[1][c] : (q == 2 ? M4[2][c] : M4[3][c]));
+  }
+  const uint32_t jj = j < PT ? j : PT - 1;
+  x = coop_m_ext(x, j, m4row);
+#pragma unroll 1
+  for (int r = 0; r < HALF_FULL; ++r) x = coop_m_ext(sbox(gl::add(x, t.p2_full[r][jj])), j, m4row);
+#pragma unroll 1
+  for (int k = 0; k < PARTIAL; ++k) {
+    const uint64_t s = sbox(gl::add(x, t.p2_partial[k]));
+    x = coop_m_int(j == 0 ? s : x, j);
+  }
+#pragma unroll 1
+  for (int r = 0; r < HALF_FULL; ++r) x = coop_m_ext(sbox(gl::add(x, t.p2_full[HALF_FULL + r][jj])), j, m4row);
+}
+
+template <int H>
+__global__ __launch_bounds__(256) void poseidon_nodes_coop_kernel(const uint64_t *__restrict__ values,
+                                                                  uint64_t *__restrict__ results, uint32_t count) {
+  __shared__ PosTable lt;
+  {
+    const uint64_t *src = (const uint64_t *)&pos_table;
+    uint64_t *dst = (uint64_t *)&lt;
+    for (uint32_t i = threadIdx.x; i < sizeof(PosTable) / sizeof(uint64_t); i += blockDim.x) dst[i] = src[i];
+  }
+  __syncthreads();
+  const uint32_t g = blockIdx.x * (blockDim.x / COOP_LANES) + threadIdx.x / COOP_LANES;
+  const uint32_t j = threadIdx.x % COOP_LANES;
+  if (g >= count) return;  // whole groups leave together
+  // element j: left[j] (j < 4), right[j - 4] (4 <= j < 8), capacity 0
+  uint64_t x = 0;
+  if (j < 2 * CAP) x = gl::canon(values[2 * ((uint64_t)g + (uint64_t)(j & 3u) * count) + (j >> 2)]);
+  if constexpr (H == 0) coop_permute(x, j, lt);
+  else coop_permute2(x, j, lt);
+  if (j < CAP) results[g + (uint64_t)j * count] = x;
+}
+
 // ---- host: the fused form ---------------------------------------------------
 
 using Mat = std::vector<uint64_t>;  // row-major n x n
@@ -396,7 +531,13 @@ int launch_leaves(const uint64_t *values, uint64_t *results, uint32_t rows, uint
 
 template <int H> int launch_nodes(const uint64_t *values, uint64_t *results, uint32_t count, hipStream_t s) {
   if (count == 0) return BC_SUCCESS;
-  poseidon_nodes_kernel<H><<<dim3((count + 255) / 256), dim3(256), 0, s>>>(values, results, count);
+  if (count <= COOP_MAX_NODES) {
+    constexpr uint32_t per_block = 256 / COOP_LANES;
+    poseidon_nodes_coop_kernel<H><<<dim3((count + per_block - 1) / per_block), dim3(256), 0, s>>>(values, results,
+                                                                                                 count);
+  } else {
+    poseidon_nodes_kernel<H><<<dim3((count + 255) / 256), dim3(256), 0, s>>>(values, results, count);
+  }
   return hipGetLastError();
 }
 

@@ -99,6 +99,20 @@ def test_nodes_match_oracle(consts, p, count):
 
 
 @pytest.mark.parametrize("p", HASHES)
+@pytest.mark.parametrize("count", [16383, 16384, 16385, 20000])
+def test_nodes_both_kernels(consts, p, count):
+    """Layers of at most 2^14 nodes run the cooperative kernel (16 lanes per
+    state), larger ones one lane per state: both against the oracle, on either
+    side of the switch."""
+    import torch
+    from boojum_cuda_amd import poseidon
+    vals = np.random.default_rng(count + 1).integers(0, 2**64, 8 * count, dtype=np.uint64)
+    res = torch.zeros(4 * count, dtype=torch.int64, device="cuda")
+    poseidon.launch_single_thread_nodes_kernel(to_dev(vals), res, p=p)
+    assert np.array_equal(to_host(res), oracle_nodes(vals, count, consts, p))
+
+
+@pytest.mark.parametrize("p", HASHES)
 @pytest.mark.parametrize("log_leaves,log_rows,cols", [(0, 0, 3), (4, 0, 8), (10, 1, 5), (12, 0, 16)])
 def test_merkle_tree_matches_oracle(consts, p, log_leaves, log_rows, cols):
     """src/poseidon.rs:628-694 (merkle_tree): every layer of the tree."""
