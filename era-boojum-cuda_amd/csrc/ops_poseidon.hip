@@ -281,7 +281,7 @@ __global__ __launch_bounds__(256) void poseidon_nodes_kernel(const uint64_t *__r
 // rows gather the other elements with __shfl inside the group, and the
 // constants are staged in LDS.  Used for layers of at most COOP_MAX_NODES
 // nodes (the reference likewise switches to cooperative kernels for small
-// layers, src/poseidon.rs:273-284).
+// layers at the same size, src/poseidon.rs:71,279-283).
 constexpr uint32_t COOP_LANES = 16, COOP_MAX_NODES = 1u << 14;
 
 __device__ __forceinline__ uint64_t grp(uint64_t v, uint32_t lane) { return __shfl(v, (int)lane, (int)COOP_LANES); }

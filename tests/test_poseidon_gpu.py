@@ -135,6 +135,45 @@ def test_merkle_tree_matches_oracle(consts, p, log_leaves, log_rows, cols):
 
 
 @pytest.mark.parametrize("p", HASHES)
+def test_merkle_tree_large_sampled(consts, p):
+    """src/poseidon.rs:690-694 (merkle_tree_poseidon2_large: 2^(14 + 3)
+    leaves, so layers use both node kernels): 64 sampled nodes of every layer
+    against the oracle applied to their children, and sampled leaves."""
+    import torch
+    from boojum_cuda_amd import poseidon
+    log_leaves = 17
+    leaves = 1 << log_leaves
+    g = torch.Generator(device="cuda")
+    g.manual_seed(17)
+    vals = torch.randint(-2**63, 2**63 - 1, (leaves,), dtype=torch.int64, device="cuda", generator=g)
+    res = torch.zeros(8 * leaves, dtype=torch.int64, device="cuda")
+    poseidon.build_merkle_tree(vals, res, 0, layers_count=log_leaves + 1, p=p)
+    got = to_host(res)
+    host = to_host(vals)
+    rng = np.random.default_rng(3)
+    pick = rng.choice(leaves, 64, replace=False)
+    want = oracle_leaves(host[pick], 1, 1, 64, consts, p)
+    for i in range(4):
+        assert np.array_equal(got[i * leaves + pick], want[i * 64:(i + 1) * 64])
+    off, n = 0, leaves
+    while n > 1:
+        below, n_below = got[off:off + 4 * n], n
+        off += 4 * n
+        n //= 2
+        layer = got[off:off + 4 * n]
+        pick = np.sort(rng.choice(n, min(n, 64), replace=False))
+        k = pick.size
+        # the picked nodes' children as a k-node layer: values[2 (g + i k) + j]
+        children = np.empty(8 * k, dtype=np.uint64)
+        for i in range(4):
+            for j in range(2):
+                children[2 * (np.arange(k) + i * k) + j] = below[i * n_below + 2 * pick + j]
+        want = oracle_nodes(children, k, consts, p)
+        for i in range(4):
+            assert np.array_equal(layer[i * n + pick], want[i * k:(i + 1) * k]), (n, i)
+
+
+@pytest.mark.parametrize("p", HASHES)
 def test_leaves_large_sampled(consts, p):
     """2^20 leaves x 8 columns (one 8-element chunk each), 64 sampled leaves
     against the oracle, plus a second call giving the same digests."""
