@@ -58,6 +58,8 @@ SIGNATURES = {
     "bc_poseidon2_nodes": (c_int, [c_vp, c_u64, c_vp, c_u64, c_vp]),
     "bc_poseidon2_build_merkle_tree_nodes": (c_int, [c_vp, c_u64, c_vp, c_u64, c_u32, c_vp]),
     "bc_poseidon2_build_merkle_tree": (c_int, [c_vp, c_u64, c_vp, c_u64, c_u32, c_u32, c_vp]),
+    "bc_gather_rows": (c_int, [c_vp, c_u32, c_u32, c_vp, c_u64, c_u64, c_vp, c_u64, c_u64, c_u32, c_vp]),
+    "bc_gather_merkle_paths": (c_int, [c_vp, c_u32, c_vp, c_u64, c_vp, c_u64, c_u32, c_vp]),
     "bc_fill_seeded": (c_int, [c_vp, c_u32, c_u32, c_u64, c_u64, c_u32, c_vp]),
     "bc_server_create": (c_int, [c_vp, ctypes.POINTER(c_vp)]),
     "bc_server_submit": (c_int, [c_vp, c_vp, ctypes.POINTER(c_u64)]),

@@ -112,3 +112,22 @@ def build_merkle_tree(values, results, log_rows_per_hash, stream=None, layers_co
     _check(_fn(p, "build_merkle_tree")(values.data_ptr(), values.numel(), results.data_ptr(),
                                        results.numel(), log_rows_per_hash, layers_count, _stream(stream)),
            "build_merkle_tree")
+
+
+def gather_rows(indexes, log_rows_per_index, values, values_rows, result, result_rows, stream=None):
+    """src/poseidon.rs:330-358.  indexes: int32/uint32 tensor; values / result:
+    column-major matrices (flat tensors, column stride = rows) with the same
+    number of columns."""
+    assert values_rows and result_rows and values.numel() % values_rows == 0 and result.numel() % result_rows == 0
+    cols = values.numel() // values_rows
+    assert result.numel() // result_rows == cols
+    _check(lib().bc_gather_rows(indexes.data_ptr(), indexes.numel(), log_rows_per_index, values.data_ptr(),
+                                values_rows, values_rows, result.data_ptr(), result_rows, result_rows, cols,
+                                _stream(stream)), "gather_rows")
+
+
+def gather_merkle_paths(indexes, values, results, layers_count, stream=None):
+    """src/poseidon.rs:360-390."""
+    _check(lib().bc_gather_merkle_paths(indexes.data_ptr(), indexes.numel(), values.data_ptr(), values.numel(),
+                                        results.data_ptr(), results.numel(), layers_count, _stream(stream)),
+           "gather_merkle_paths")

@@ -408,6 +408,40 @@ __global__ __launch_bounds__(256) void poseidon_nodes_coop_kernel(const uint64_t
   if (j < CAP) results[g + (uint64_t)j * count] = x;
 }
 
+// ---- Merkle openings: row and path gathers (native/poseidon_common.cu:9-37) --
+
+// result[col][i rpi + r] = values[col][indexes[i] rpi + r] for r < rpi =
+// 2^log_rpi; rows past the source (an index out of range) are skipped, the
+// destination left as it was.  x: destination rows, y: columns.
+__global__ __launch_bounds__(256) void gather_rows_kernel(const uint32_t *__restrict__ indexes, uint64_t dst_rows,
+                                                          uint32_t log_rpi, const uint64_t *__restrict__ values,
+                                                          uint64_t values_stride, uint64_t values_rows,
+                                                          uint64_t *__restrict__ results, uint64_t results_stride,
+                                                          uint32_t cols) {
+  const uint64_t dst = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (dst >= dst_rows) return;
+  const uint64_t src = ((uint64_t)indexes[dst >> log_rpi] << log_rpi) + (dst & ((1ull << log_rpi) - 1));
+  if (src >= values_rows) return;
+  for (uint32_t c = blockIdx.y; c < cols; c += gridDim.y) results[c * results_stride + dst] = values[c * values_stride + src];
+}
+
+// Siblings along each index's path: for layer l < layers, column c (of 4) and
+// index i, results[l count 4 + c count + i] = the layer-l digest element c of
+// node (indexes[i] >> l) ^ 1.  Layer l of the tree buffer starts at
+// 4 (2^(L+1) - 2^(L+1-l)) and holds 4 columns of 2^(L-l) (L = log2 leaves).
+__global__ __launch_bounds__(256) void gather_merkle_paths_kernel(const uint32_t *__restrict__ indexes,
+                                                                  uint32_t count, const uint64_t *__restrict__ values,
+                                                                  uint32_t log_leaves, uint64_t *__restrict__ results) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const uint32_t c = blockIdx.y, l = blockIdx.z;
+  const uint64_t leaf = indexes[i];
+  if (leaf >> log_leaves) return;  // index out of range: nothing to gather
+  const uint64_t layer_off = ((uint64_t)CAP << (log_leaves + 1)) - ((uint64_t)CAP << (log_leaves + 1 - l));
+  const uint64_t src = layer_off + ((uint64_t)c << (log_leaves - l)) + ((leaf >> l) ^ 1ull);
+  results[(uint64_t)l * count * CAP + (uint64_t)count * c + i] = values[src];
+}
+
 // ---- host: the fused form ---------------------------------------------------
 
 using Mat = std::vector<uint64_t>;  // row-major n x n
@@ -696,4 +730,41 @@ extern "C" int bc_poseidon2_build_merkle_tree(const uint64_t *values, uint64_t v
                                               uint64_t results_len, uint32_t log_rows_per_hash,
                                               uint32_t layers_count, void *stream) {
   return bc::tree_impl<1>(values, values_len, results, results_len, log_rows_per_hash, layers_count, stream);
+}
+
+extern "C" int bc_gather_rows(const uint32_t *indexes, uint32_t indexes_count, uint32_t log_rows_per_index,
+                              const uint64_t *values, uint64_t values_stride, uint64_t values_rows, uint64_t *results,
+                              uint64_t results_stride, uint64_t results_rows, uint32_t cols, void *stream) {
+  using namespace bc;
+  // src/poseidon.rs:337-346
+  if (log_rows_per_index >= 32) return BC_ERROR_INVALID_VALUE;
+  if (results_rows != ((uint64_t)indexes_count << log_rows_per_index)) return BC_ERROR_INVALID_VALUE;
+  if (results_rows == 0 || cols == 0) return BC_SUCCESS;
+  if (!indexes || !values || !results || values_stride < values_rows || results_stride < results_rows)
+    return BC_ERROR_INVALID_VALUE;
+  const uint64_t blocks = (results_rows + 255) / 256;
+  if (blocks > 0x7FFFFFFFull) return BC_ERROR_INVALID_VALUE;
+  gather_rows_kernel<<<dim3((uint32_t)blocks, cols < 65535 ? cols : 65535), dim3(256), 0, (hipStream_t)stream>>>(
+      indexes, results_rows, log_rows_per_index, values, values_stride, values_rows, results, results_stride, cols);
+  return hipGetLastError();
+}
+
+extern "C" int bc_gather_merkle_paths(const uint32_t *indexes, uint32_t indexes_count, const uint64_t *values,
+                                      uint64_t values_len, uint64_t *results, uint64_t results_len,
+                                      uint32_t layers_count, void *stream) {
+  using namespace bc;
+  // src/poseidon.rs:367-382
+  if (values_len % CAP != 0) return BC_ERROR_INVALID_VALUE;
+  const uint64_t values_count = values_len / CAP;
+  if (values_count == 0 || (values_count & (values_count - 1)) != 0) return BC_ERROR_INVALID_VALUE;
+  const uint32_t log_values = (uint32_t)__builtin_ctzll(values_count);
+  if (log_values == 0) return BC_ERROR_INVALID_VALUE;
+  const uint32_t log_leaves = log_values - 1;
+  if (layers_count >= log_leaves) return BC_ERROR_INVALID_VALUE;
+  if ((uint64_t)indexes_count * layers_count * CAP != results_len) return BC_ERROR_INVALID_VALUE;
+  if (indexes_count == 0 || layers_count == 0) return BC_SUCCESS;
+  if (!indexes || !values || !results) return BC_ERROR_INVALID_VALUE;
+  gather_merkle_paths_kernel<<<dim3((indexes_count + 255) / 256, CAP, layers_count), dim3(256), 0,
+                               (hipStream_t)stream>>>(indexes, indexes_count, values, log_leaves, results);
+  return hipGetLastError();
 }

@@ -297,6 +297,28 @@ int bc_poseidon2_build_merkle_tree(const uint64_t *values, uint64_t values_len, 
                                    uint64_t results_len, uint32_t log_rows_per_hash, uint32_t layers_count,
                                    void *stream);
 
+/* gather_rows (src/poseidon.rs:330-358): for each of indexes_count indexes,
+ * its 2^log_rows_per_index rows (index * 2^log_rows_per_index + r) of every
+ * column of the column-major matrix values (column c at values + c *
+ * values_stride, values_rows rows) go to rows i * 2^log_rows_per_index + r of
+ * results (column c at results + c * results_stride, results_rows =
+ * indexes_count << log_rows_per_index).  Unlike the reference, a source row
+ * at or past values_rows is not read and its destination row is left as it
+ * was. */
+int bc_gather_rows(const uint32_t *indexes, uint32_t indexes_count, uint32_t log_rows_per_index,
+                   const uint64_t *values, uint64_t values_stride, uint64_t values_rows, uint64_t *results,
+                   uint64_t results_stride, uint64_t results_rows, uint32_t cols, void *stream);
+
+/* gather_merkle_paths (src/poseidon.rs:360-390): values = a whole tree as
+ * bc_poseidon_build_merkle_tree writes it (4 2^(L+1) elements for 2^L
+ * leaves); results = layers_count layers of 4 columns of indexes_count
+ * siblings: layer l, column c, index i gets element c of node
+ * (indexes[i] >> l) ^ 1 of layer l.  layers_count < L.  An index at or past
+ * 2^L gathers nothing (the reference reads out of range). */
+int bc_gather_merkle_paths(const uint32_t *indexes, uint32_t indexes_count, const uint64_t *values,
+                           uint64_t values_len, uint64_t *results, uint64_t results_len, uint32_t layers_count,
+                           void *stream);
+
 /* Seeded inputs on the device: column c (c = 0..num_cols-1) at
  * out + c * stride gets 2^log_n u64 draws of the splitmix64 stream seeded with
  * seed * 0x9E3779B97F4A7C15 + first_col + c (raw u64, any value, read mod p).

@@ -228,3 +228,76 @@ def test_hashes_differ(consts):
     poseidon.launch_single_thread_leaves_kernel(vals, a, 0, p="poseidon")
     poseidon.launch_single_thread_leaves_kernel(vals, b, 0, p="poseidon2")
     assert not torch.equal(a, b)
+
+
+# ---- Merkle openings: gather_rows / gather_merkle_paths -------------------------
+
+def idx_dev(a):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=np.uint32).view(np.int32)).to("cuda")
+
+
+@pytest.mark.parametrize("log_rpi,index_hi", [(1, 42), (1, 2048), (0, 4096), (3, 512)])
+def test_gather_rows(gpu_ctx, log_rpi, index_hi):
+    """src/poseidon.rs:817-862 (gather_rows): 16 columns of 2^12 rows, 42
+    indexes, each index's 2^log_rpi rows of every column."""
+    import torch
+    from boojum_cuda_amd import poseidon
+    src_rows, cols, count = 1 << 12, 16, 42
+    rng = np.random.default_rng(log_rpi * 1000 + index_hi)
+    idx = rng.integers(0, index_hi, count).astype(np.uint32)
+    vals = rng.integers(0, 2**64, src_rows * cols, dtype=np.uint64)
+    dst_rows = count << log_rpi
+    res = torch.zeros(dst_rows * cols, dtype=torch.int64, device="cuda")
+    poseidon.gather_rows(idx_dev(idx), log_rpi, to_dev(vals), src_rows, res, dst_rows)
+    got = to_host(res)
+    for i, index in enumerate(idx):
+        for j in range(1 << log_rpi):
+            for k in range(cols):
+                assert got[k * dst_rows + (i << log_rpi) + j] == vals[(k << 12) + (int(index) << log_rpi) + j]
+
+
+def test_gather_rows_skips_out_of_range(gpu_ctx):
+    import torch
+    from boojum_cuda_amd import poseidon
+    vals = np.arange(64 * 2, dtype=np.uint64) + 100
+    idx = np.array([3, 64, 10, 1 << 31], dtype=np.uint32)  # rows 64 and 2^31 do not exist
+    res = torch.full((4 * 2,), 7, dtype=torch.int64, device="cuda")
+    poseidon.gather_rows(idx_dev(idx), 0, to_dev(vals), 64, res, 4)
+    got = to_host(res)
+    assert got.tolist() == [103, 7, 110, 7, 167, 7, 174, 7]
+
+
+@pytest.mark.parametrize("log_leaves,layers", [(12, 8), (5, 4), (2, 1)])
+def test_gather_merkle_paths(gpu_ctx, log_leaves, layers):
+    """src/poseidon.rs:864-916 (gather_merkle_paths): the sibling of each
+    index's node on every layer, checked by the reference's recursion."""
+    import torch
+    from boojum_cuda_amd import poseidon
+    count = 42
+    rng = np.random.default_rng(log_leaves)
+    idx = rng.integers(0, 1 << log_leaves, count).astype(np.uint32)
+    vals = rng.integers(0, 2**64, 4 << (log_leaves + 1), dtype=np.uint64)
+    res = torch.zeros(4 * count * layers, dtype=torch.int64, device="cuda")
+    poseidon.gather_merkle_paths(idx_dev(idx), to_dev(vals), res, layers)
+    got = to_host(res)
+    values, results, ix = vals, got, idx.astype(np.int64)
+    for _ in range(layers):
+        half = values.size // 2
+        layer, values = values[:half], values[half:]
+        cur, results = results[:count * 4], results[count * 4:]
+        n = layer.size // 4
+        for c in range(4):
+            assert np.array_equal(cur[c * count:(c + 1) * count], layer[c * n + (ix ^ 1)])
+        ix = ix >> 1
+
+
+def test_gather_bad_arguments_raise(gpu_ctx):
+    import torch
+    from boojum_cuda_amd import poseidon
+    idx = idx_dev(np.zeros(4, dtype=np.uint32))
+    v = torch.zeros(64, dtype=torch.int64, device="cuda")
+    with pytest.raises(AssertionError):  # result rows != indexes << log_rpi
+        poseidon.gather_rows(idx, 1, v, 16, torch.zeros(12, dtype=torch.int64, device="cuda"), 3)
+    with pytest.raises(AssertionError):  # layers_count must be below log2(leaves)
+        poseidon.gather_merkle_paths(idx, v, torch.zeros(4 * 4 * 3, dtype=torch.int64, device="cuda"), 3)

@@ -88,3 +88,20 @@ def test_tree_nodes_rejects_reference_asserts(values_len, results_len, layers):
 def test_tree_rejects_reference_asserts(values_len, results_len, layers):
     assert lib().bc_poseidon_build_merkle_tree(1, values_len, 1, results_len, 0, layers, None) == \
         BC_ERROR_INVALID_VALUE
+
+
+@pytest.mark.parametrize("count,log_rpi,results_rows", [(4, 32, 4), (4, 1, 4), (3, 0, 4)])
+def test_gather_rows_rejects_reference_asserts(count, log_rpi, results_rows):
+    assert lib().bc_gather_rows(1, count, log_rpi, 1, 16, 16, 1, results_rows, results_rows, 2, None) == \
+        BC_ERROR_INVALID_VALUE
+
+
+@pytest.mark.parametrize("values_len,results_len,layers", [
+    (30, 16, 1),    # not a multiple of the capacity
+    (24, 16, 1),    # 6 digests: not a power of two
+    (4, 0, 0),      # one digest: no leaf layer
+    (64, 16 * 3, 3),  # 8 leaves: layers must be below 3
+    (64, 20, 1),    # results != indexes * layers * 4
+])
+def test_gather_merkle_paths_rejects_reference_asserts(values_len, results_len, layers):
+    assert lib().bc_gather_merkle_paths(1, 4, 1, values_len, 1, results_len, layers, None) == BC_ERROR_INVALID_VALUE
