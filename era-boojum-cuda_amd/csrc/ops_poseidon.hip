@@ -669,6 +669,10 @@ extern "C" int bc_poseidon_set_round_constants(const uint64_t *round_constants) 
   PosTable t{};
   if (!derive(round_constants, t)) return BC_ERROR_INVALID_VALUE;
   std::lock_guard<std::mutex> lk(pos_mu);
+  // hashing kernels already queued on any stream of this device read the
+  // table: let them finish before it changes
+  err = hipDeviceSynchronize();
+  if (err != hipSuccess) return err;
   err = hipMemcpyToSymbol(HIP_SYMBOL(pos_table), &t, sizeof t, 0, hipMemcpyHostToDevice);
   if (err != hipSuccess) return err;
   err = hipDeviceSynchronize();
