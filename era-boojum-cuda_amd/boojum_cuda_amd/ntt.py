@@ -7,6 +7,8 @@ the current stream), standing in for `CudaStream`.  Argument violations raise
 AssertionError before anything is launched, as the reference's `assert!`s
 panic; launch failures raise HipError (the reference's `CudaResult::Err`).
 """
+import ctypes
+
 from ._lib import BC_ERROR_INVALID_VALUE, BC_ERROR_NOT_INITIALIZED, HipError, lib
 
 
@@ -69,3 +71,21 @@ def batch_ntt_in_place(inputs_matrix, log_n, num_ntts, inputs_offset, stride_bet
                                      bool(bitrev_inputs), bool(inverse), log_extension_degree, coset_index,
                                      _stream(inputs_matrix, stream))
     _check(rc, "batch_ntt_in_place")
+
+
+class _PassInfo(ctypes.Structure):
+    _fields_ = [("contiguous", ctypes.c_uint32), ("log_h", ctypes.c_uint32), ("log_block", ctypes.c_uint32),
+                ("mode", ctypes.c_uint32), ("work_items", ctypes.c_uint64), ("positions", ctypes.c_uint64),
+                ("groups", ctypes.c_uint64), ("pairs_per_workgroup", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+
+
+def describe_plan(log_n, num_ntts, bitrev_inputs=False, inverse=False, log_extension_degree=0, coset_index=0):
+    """The HBM passes batch_ntt_internal launches for this shape, as dicts
+    (host only, no device needed).  Empty for log_n < 8 (one generic pass)."""
+    arr = (_PassInfo * 8)()
+    n = ctypes.c_uint32()
+    rc = lib().bc_ntt_describe_plan(log_n, num_ntts, bool(bitrev_inputs), bool(inverse), log_extension_degree,
+                                    coset_index, arr, 8, ctypes.byref(n))
+    _check(rc, "ntt_describe_plan")
+    return [{name: getattr(arr[i], name) for name, _ in _PassInfo._fields_ if name != "reserved"}
+            for i in range(n.value)]

@@ -74,12 +74,6 @@ class RaggedPlan:
         s = (stream or torch.cuda.current_stream()).cuda_stream
         _check(lib().bc_ragged_run(self._h, s), "ragged_run")
 
-    def failed(self):
-        """True if a unit gave up waiting for its predecessor (call after the stream completed)."""
-        f = ctypes.c_int()
-        _check(lib().bc_ragged_status(self._h, ctypes.byref(f)), "ragged_status")
-        return bool(f.value)
-
     def info(self):
         u, lv, wg = ctypes.c_uint64(), ctypes.c_uint32(), ctypes.c_uint32()
         _check(lib().bc_ragged_plan_info(self._h, ctypes.byref(u), ctypes.byref(lv), ctypes.byref(wg)), "plan_info")

@@ -39,6 +39,24 @@ class Arena {
     return false;
   }
 
+  // The top end of the highest-addressed free range that holds `bytes`;
+  // false if none does.  Long-lived buffers placed this way stay clear of
+  // the low end, where short-lived ones come and go.
+  bool alloc_high(uint64_t bytes, uint64_t *off) {
+    const uint64_t need = round(bytes ? bytes : 1);
+    for (auto it = free_.rbegin(); it != free_.rend(); ++it) {
+      if (it->second < need) continue;
+      const uint64_t rest = it->second - need;
+      *off = it->first + rest;
+      if (rest) it->second = rest;
+      else free_.erase(std::next(it).base());
+      used_ += need;
+      if (used_ > high_) high_ = used_;
+      return true;
+    }
+    return false;
+  }
+
   // Returns [off, off + round(bytes)) to the free list.  The range must be
   // allocated (a whole allocation or an aligned piece of one).
   void release(uint64_t off, uint64_t bytes) {

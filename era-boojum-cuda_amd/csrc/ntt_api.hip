@@ -100,6 +100,18 @@ int run_general(bc_context *ctx, const uint64_t *inputs, uint64_t *outputs, uint
 
 }  // namespace
 
+uint32_t strided_groups(uint64_t pairs, uint64_t positions, uint32_t tuned_bpw, uint32_t *bpw_out) {
+  // about 2048 (r-range, pair group) workgroups, 1..16 pairs each, unless tuned
+  uint64_t bpw = pairs * positions / 2048;
+  bpw = bpw < 1 ? 1 : (bpw > 16 ? 16 : bpw);
+  if (tuned_bpw) bpw = tuned_bpw;
+  // grid.y is at most 65535: larger batches take more pairs per workgroup
+  const uint64_t min_bpw = (pairs + STRIDED_MAX_GROUPS - 1) / STRIDED_MAX_GROUPS;
+  if (bpw < min_bpw) bpw = min_bpw;
+  *bpw_out = (uint32_t)bpw;
+  return (uint32_t)((pairs + bpw - 1) / bpw);
+}
+
 Tables current_tables(bool &ok) {
   bc_context *ctx = current_context();
   ok = ctx != nullptr;
@@ -282,21 +294,22 @@ extern "C" int bc_batch_ntt_internal(const uint64_t *inputs, uint64_t *outputs, 
                                       bitrev_inputs, inverse, log_ext, coset_index, passes);
   if (np <= 0) return BC_ERROR_INVALID_VALUE;
   const TuneParams tp = tuned(log_n, num_ntts);
+  // every pass's launch shape is fixed before the first launch, so a shape
+  // that cannot be launched is refused with nothing queued (an in-place
+  // transform would otherwise lose its input to the passes already run)
+  uint32_t groups[8] = {};
   for (int k = 0; k < np; ++k) {
     PassDesc &pd = passes[k];
-    hipError_t err;
-    if (pd.contig) {
-      err = launch_contig(pd.a, (int)pd.log_h, inverse, bitrev_inputs, pd.mode, tp.contig_grid, s);
-    } else {
-      uint64_t bpw = pd.pairs * pd.positions / 2048;
-      bpw = bpw < 1 ? 1 : (bpw > 16 ? 16 : bpw);
-      if (tp.strided_bpw) bpw = tp.strided_bpw;
-      const uint64_t groups = (pd.pairs + bpw - 1) / bpw;
-      if (pd.positions > 0x7FFFFFFFull || groups > 65535) return BC_ERROR_INVALID_VALUE;
-      pd.a.blocks_per_wg = (uint32_t)bpw;
-      err = launch_strided(pd.a, (int)pd.log_h, inverse, bitrev_inputs, pd.mode, (uint32_t)pd.positions,
-                           (uint32_t)groups, s);
-    }
+    if (pd.contig) continue;
+    if (pd.positions > 0x7FFFFFFFull) return BC_ERROR_INVALID_VALUE;
+    groups[k] = strided_groups(pd.pairs, pd.positions, tp.strided_bpw, &pd.a.blocks_per_wg);
+  }
+  for (int k = 0; k < np; ++k) {
+    const PassDesc &pd = passes[k];
+    const hipError_t err =
+        pd.contig ? launch_contig(pd.a, (int)pd.log_h, inverse, bitrev_inputs, pd.mode, tp.contig_grid, s)
+                  : launch_strided(pd.a, (int)pd.log_h, inverse, bitrev_inputs, pd.mode, (uint32_t)pd.positions,
+                                   groups[k], s);
     if (err != hipSuccess) return err;
   }
   return BC_SUCCESS;
@@ -423,4 +436,39 @@ extern "C" int bc_graph_destroy(bc_graph *graph) {
   hipError_t e2 = hipGraphDestroy(graph->graph);
   delete graph;
   return e1 != hipSuccess ? e1 : e2;
+}
+
+extern "C" int bc_ntt_describe_plan(uint32_t log_n, uint32_t num_ntts, bool bitrev_inputs, bool inverse,
+                                    uint32_t log_ext, uint32_t coset_index, bc_pass_info *passes, uint32_t max_passes,
+                                    uint32_t *num_passes) {
+  using namespace bc;
+  if (!num_passes || log_n < 1 || log_n >= 32) return BC_ERROR_INVALID_VALUE;
+  if ((uint64_t)log_n + log_ext > OMEGA_LOG_ORDER) return BC_ERROR_INVALID_VALUE;
+  if (coset_index > 0 && (log_ext == 0 || (uint64_t)coset_index >= (1ull << log_ext))) return BC_ERROR_INVALID_VALUE;
+  *num_passes = 0;
+  if (log_n < 8 || num_ntts == 0) return BC_SUCCESS;
+  PassDesc pd[8];
+  const int np = describe_fast_passes(Tables{}, nullptr, nullptr, log_n, num_ntts, 1ull << log_n, 1ull << log_n,
+                                      bitrev_inputs, inverse, log_ext, coset_index, pd);
+  if (np <= 0) return BC_ERROR_INVALID_VALUE;
+  if (!passes || max_passes < (uint32_t)np) return BC_ERROR_INVALID_VALUE;
+  const TuneParams tp = tuned(log_n, num_ntts);
+  for (int k = 0; k < np; ++k) {
+    bc_pass_info &o = passes[k];
+    o = bc_pass_info{};
+    o.contiguous = pd[k].contig ? 1u : 0u;
+    o.log_h = pd[k].log_h;
+    o.log_block = pd[k].a.log_N;
+    o.mode = (uint32_t)pd[k].mode;
+    if (pd[k].contig) {
+      o.work_items = pd[k].a.tiles;
+    } else {
+      if (pd[k].positions > 0x7FFFFFFFull) return BC_ERROR_INVALID_VALUE;
+      o.work_items = pd[k].pairs;
+      o.positions = pd[k].positions;
+      o.groups = strided_groups(pd[k].pairs, pd[k].positions, tp.strided_bpw, &o.pairs_per_workgroup);
+    }
+  }
+  *num_passes = (uint32_t)np;
+  return BC_SUCCESS;
 }

@@ -110,6 +110,13 @@ int describe_fast_passes(const Tables &tab, const uint64_t *inputs, uint64_t *ou
                          uint32_t num_cols, uint64_t stride_in, uint64_t stride_out, bool bitrev_inputs, bool inverse,
                          uint32_t log_ext, uint32_t coset_index, PassDesc *out);
 
+// Launch shape of a strided pass over `pairs` (column, block) pairs at
+// `positions` r-ranges: returns the pair groups (grid.y, at most
+// STRIDED_MAX_GROUPS) and stores the pairs per workgroup in *bpw.  tuned_bpw
+// (0: built-in) is raised when the groups would not fit the grid.
+constexpr uint64_t STRIDED_MAX_GROUPS = 65535;
+uint32_t strided_groups(uint64_t pairs, uint64_t positions, uint32_t tuned_bpw, uint32_t *bpw);
+
 // max_wgs: grid cap of a contiguous pass (0: built-in 2048).
 hipError_t launch_contig(const FastArgs &a, int log_h, bool inverse, bool dit, int mode, uint32_t max_wgs,
                          hipStream_t stream);

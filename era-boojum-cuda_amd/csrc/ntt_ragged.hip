@@ -120,14 +120,11 @@ __device__ __forceinline__ void run_unit(const RaggedSeg *sg, uint32_t variant, 
 
 struct QueueState {
   unsigned long long head[RAGGED_MAX_LEVELS];  // units popped so far, per level
-  uint32_t error;                              // stays 0 (see bc_ragged_status)
-  uint32_t pad;
 };
 
 __global__ void ragged_reset_kernel(QueueState *st) {
   const uint32_t i = threadIdx.x;
   if (i < RAGGED_MAX_LEVELS) st->head[i] = 0;
-  if (i == 0) st->error = 0;
 }
 
 // One launch drains the units of one dependency level: segments
@@ -412,12 +409,6 @@ hipError_t ragged_launch(const void *d_table, const RaggedHostPlan &hp, void *d_
   return err;
 }
 
-uint32_t *ragged_error_ptr(void *d_state) { return &((QueueState *)d_state)->error; }
-
-hipError_t ragged_read_error(const void *d_state, uint32_t *error) {
-  return hipMemcpy(error, &((const QueueState *)d_state)->error, sizeof(uint32_t), hipMemcpyDeviceToHost);
-}
-
 }  // namespace bc
 
 struct bc_ragged_plan {
@@ -471,15 +462,6 @@ extern "C" int bc_ragged_plan_create(const bc_ragged_job *jobs, uint32_t num_job
 extern "C" int bc_ragged_run(bc_ragged_plan *p, void *stream) {
   if (!p) return BC_ERROR_INVALID_VALUE;
   return bc::ragged_launch(p->d_table, p->hp, p->d_state, p->workgroups, (hipStream_t)stream);
-}
-
-extern "C" int bc_ragged_status(bc_ragged_plan *p, int *failed) {
-  if (!p || !failed) return BC_ERROR_INVALID_VALUE;
-  uint32_t e = 0;
-  hipError_t err = bc::ragged_read_error(p->d_state, &e);
-  if (err != hipSuccess) return err;
-  *failed = e ? 1 : 0;
-  return BC_SUCCESS;
 }
 
 extern "C" int bc_ragged_plan_info(const bc_ragged_plan *p, uint64_t *units, uint32_t *levels, uint32_t *workgroups) {

