@@ -28,6 +28,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "era-boojum-cuda_amd"))
 
 HBM_PEAK_GBS = 8000.0
+CPU_SHARE = 16  # host cores that come with one GPU of the node
 METRIC = "Goldilocks NTT/LDE elems/s & %HBM peak; ragged-batched vs plain; p50/p99 latency"
 
 
@@ -354,29 +355,37 @@ def cpu_model():
     return "unknown"
 
 
-def cpu_baseline(log_n, cols, threads):
+def cpu_baseline(log_n, cols, threads, warmup=3, runs=10):
+    """The oracle's multithreaded NTT (threads over columns) on the full
+    matrix: `warmup` untimed calls, then the median of `runs` timed calls."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import numpy as np
     import oracle
     n = 1 << log_n
     inp = oracle.matrix_uniform(cols, n, 2)
     out = np.empty_like(inp)
-    t0 = time.perf_counter()
-    oracle.batch_ntt(inp, log_n, cols, n, n, False, False, out=out, threads=threads)
-    dt = time.perf_counter() - t0
-    # config 1 on one thread: 1 x 2^16 forward n2b, 3 warm-up + 15 timed runs, median
+    times = []
+    for k in range(warmup + runs):
+        t0 = time.perf_counter()
+        oracle.batch_ntt(inp, log_n, cols, n, n, False, False, out=out, threads=threads)
+        if k >= warmup:
+            times.append(time.perf_counter() - t0)
+    times.sort()
+    dt = times[len(times) // 2]
+    # 1 x 2^16 forward n2b on one thread, 3 warm-up + 15 timed runs, median
     one = oracle.matrix_uniform(1, 1 << 16, 1)
-    runs = []
+    single = []
     for k in range(18):
         t1 = time.perf_counter()
         oracle.batch_ntt(one, 16, 1, 1 << 16, 1 << 16, False, False)
         if k >= 3:
-            runs.append(time.perf_counter() - t1)
-    runs.sort()
+            single.append(time.perf_counter() - t1)
+    single.sort()
     return {"value": cols * n / dt, "unit": "elem/s", "cores": threads, "kind": "port",
-            "sample": f"{cols} columns x 2^{log_n}, forward n2b, {dt:.2f} s wall",
-            "nproc": os.cpu_count(), "cpu_model": cpu_model(),
-            "config1_single_thread_ms_median": runs[len(runs) // 2] * 1e3}
+            "sample": f"{cols} columns x 2^{log_n}, forward n2b, {warmup} warm-up + median of {runs} calls "
+                      f"({dt:.3f} s per call, min {times[0]:.3f}, max {times[-1]:.3f})",
+            "nproc": os.cpu_count(), "cpus_allowed": len(os.sched_getaffinity(0)), "cpu_model": cpu_model(),
+            "config1_single_thread_ms_median": single[len(single) // 2] * 1e3}
 
 
 def measured_traffic(log_n, cols):
@@ -545,7 +554,9 @@ def main():
     }
     rec.update(extras)
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
+        # the host cores this process may run on, at most the 16 a one-GPU
+        # share of the host provides
+        threads = min(CPU_SHARE, len(os.sched_getaffinity(0)))
         progress(f"cpu baseline on {threads} threads")
         rec["cpu_baseline"] = cpu_baseline(log_n, args.cpu_cols, threads)
     print(json.dumps(rec), flush=True)

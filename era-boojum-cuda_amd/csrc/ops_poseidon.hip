@@ -40,6 +40,7 @@
 #include <string.h>
 
 #include <mutex>
+#include <shared_mutex>
 #include <vector>
 
 namespace bc {
@@ -542,17 +543,23 @@ bool derive(const uint64_t *rc_in, PosTable &t) {
 }
 
 constexpr int MAX_DEVICES = 64;
-std::mutex pos_mu;
+// Every hashing entry point holds this lock shared from its readiness check
+// through its launches; replacing the table holds it exclusively.  So no
+// hashing kernel is queued between the wait for queued work and the copy of
+// the new table, on any stream or thread.
+std::shared_mutex pos_mu;
 bool pos_ready[MAX_DEVICES];
 
+// Caller holds pos_mu (shared or exclusive).
 int require_table() {
   int dev = 0;
   const hipError_t err = hipGetDevice(&dev);
   if (err != hipSuccess) return err;
   if (dev < 0 || dev >= MAX_DEVICES) return BC_ERROR_INVALID_VALUE;
-  std::lock_guard<std::mutex> lk(pos_mu);
   return pos_ready[dev] ? BC_SUCCESS : BC_ERROR_NOT_INITIALIZED;
 }
+
+using TableUse = std::shared_lock<std::shared_mutex>;
 
 template <int H>
 int launch_leaves(const uint64_t *values, uint64_t *results, uint32_t rows, uint32_t cols, uint32_t count, bool load,
@@ -668,9 +675,10 @@ extern "C" int bc_poseidon_set_round_constants(const uint64_t *round_constants) 
   if (dev < 0 || dev >= MAX_DEVICES) return BC_ERROR_INVALID_VALUE;
   PosTable t{};
   if (!derive(round_constants, t)) return BC_ERROR_INVALID_VALUE;
-  std::lock_guard<std::mutex> lk(pos_mu);
+  std::unique_lock<std::shared_mutex> lk(pos_mu);
   // hashing kernels already queued on any stream of this device read the
-  // table: let them finish before it changes
+  // table: let them finish before it changes (no new ones can be queued
+  // while the lock is held)
   err = hipDeviceSynchronize();
   if (err != hipSuccess) return err;
   err = hipMemcpyToSymbol(HIP_SYMBOL(pos_table), &t, sizeof t, 0, hipMemcpyHostToDevice);
@@ -693,46 +701,54 @@ extern "C" int bc_poseidon_fused_table(const uint64_t *round_constants, uint64_t
 extern "C" int bc_poseidon_leaves(const uint64_t *values, uint64_t values_len, uint64_t *results,
                                   uint64_t results_len, uint32_t log_rows_per_hash, bool load_intermediate,
                                   bool store_intermediate, void *stream) {
+  const bc::TableUse use(bc::pos_mu);
   return bc::leaves_impl<0>(values, values_len, results, results_len, log_rows_per_hash, load_intermediate,
                             store_intermediate, stream);
 }
 
 extern "C" int bc_poseidon_nodes(const uint64_t *values, uint64_t values_len, uint64_t *results,
                                  uint64_t results_len, void *stream) {
+  const bc::TableUse use(bc::pos_mu);
   return bc::nodes_impl<0>(values, values_len, results, results_len, stream);
 }
 
 extern "C" int bc_poseidon_build_merkle_tree_nodes(const uint64_t *values, uint64_t values_len, uint64_t *results,
                                                    uint64_t results_len, uint32_t layers_count, void *stream) {
+  const bc::TableUse use(bc::pos_mu);
   return bc::tree_nodes_impl<0>(values, values_len, results, results_len, layers_count, stream);
 }
 
 extern "C" int bc_poseidon_build_merkle_tree(const uint64_t *values, uint64_t values_len, uint64_t *results,
                                              uint64_t results_len, uint32_t log_rows_per_hash,
                                              uint32_t layers_count, void *stream) {
+  const bc::TableUse use(bc::pos_mu);
   return bc::tree_impl<0>(values, values_len, results, results_len, log_rows_per_hash, layers_count, stream);
 }
 
 extern "C" int bc_poseidon2_leaves(const uint64_t *values, uint64_t values_len, uint64_t *results,
                                    uint64_t results_len, uint32_t log_rows_per_hash, bool load_intermediate,
                                    bool store_intermediate, void *stream) {
+  const bc::TableUse use(bc::pos_mu);
   return bc::leaves_impl<1>(values, values_len, results, results_len, log_rows_per_hash, load_intermediate,
                             store_intermediate, stream);
 }
 
 extern "C" int bc_poseidon2_nodes(const uint64_t *values, uint64_t values_len, uint64_t *results,
                                   uint64_t results_len, void *stream) {
+  const bc::TableUse use(bc::pos_mu);
   return bc::nodes_impl<1>(values, values_len, results, results_len, stream);
 }
 
 extern "C" int bc_poseidon2_build_merkle_tree_nodes(const uint64_t *values, uint64_t values_len, uint64_t *results,
                                                     uint64_t results_len, uint32_t layers_count, void *stream) {
+  const bc::TableUse use(bc::pos_mu);
   return bc::tree_nodes_impl<1>(values, values_len, results, results_len, layers_count, stream);
 }
 
 extern "C" int bc_poseidon2_build_merkle_tree(const uint64_t *values, uint64_t values_len, uint64_t *results,
                                               uint64_t results_len, uint32_t log_rows_per_hash,
                                               uint32_t layers_count, void *stream) {
+  const bc::TableUse use(bc::pos_mu);
   return bc::tree_impl<1>(values, values_len, results, results_len, log_rows_per_hash, layers_count, stream);
 }
 

@@ -237,11 +237,11 @@ int bc_batch_eval(uint32_t kind, const uint64_t *ys, uint64_t stride, uint32_t n
  * round constants (30 rounds x 12, round-major; boojum's
  * poseidon_goldilocks_params::ALL_ROUND_CONSTANTS).  Derives the fused form
  * the kernels run (the reference's poseidon_constants_template.cuh tables)
- * and stores it on the current device.  Synchronous: it first waits for all
- * work queued on the device (hashing kernels read the table), so it must not
- * be called while another thread is enqueueing hashing work there.  Every
- * hashing call on a device returns BC_ERROR_NOT_INITIALIZED until this has
- * been called there. */
+ * and stores it on the current device.  Synchronous: it holds the library's
+ * table lock exclusively (hashing calls on other threads wait for it), waits
+ * for all work queued on the device (hashing kernels read the table), then
+ * replaces the table.  Every hashing call on a device returns
+ * BC_ERROR_NOT_INITIALIZED until this has been called there. */
 int bc_poseidon_set_round_constants(const uint64_t *round_constants);
 
 /* Host only (no device call): the per-device table derived from 360
