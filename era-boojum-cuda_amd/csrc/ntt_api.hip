@@ -1,5 +1,5 @@
 // C ABI entry points: context management and the batched NTT host logic
-// (argument checks, pass planning, launches).  Mirrors src/context.rs and
+// (argument checks and launches; the pass planning is in ntt_plan.cpp).  Mirrors src/context.rs and
 // src/ntt.rs of the reference; see include/boojum_ntt.h for the contract.
 #include "../../include/boojum_ntt.h"
 #include "gl_field.hpp"
@@ -39,44 +39,6 @@ bc_context *current_context() {
 }
 
 
-// Split of log_n >= 8 into strided passes of 4..8 bits (top index bits first)
-// and one contiguous pass of 8..12 bits: the tuned split when one is set,
-// else the fewest strided passes with the largest contiguous pass.
-struct FastPlan {
-  int num_strided;
-  uint32_t strided[8];
-  uint32_t log_N[8];  // block length of each strided pass
-  uint32_t contig;
-};
-
-void make_fast_plan(uint32_t log_n, const TuneParams &tp, FastPlan &pl) {
-  pl.num_strided = 0;
-  if (log_n <= TILE_LOG) {
-    pl.contig = log_n;
-    return;
-  }
-  uint32_t c = TILE_LOG, s = log_n - TILE_LOG;
-  uint32_t k = (s + 7) / 8;
-  if (s < 4 * k) {  // only when k == 1 and s < 4: shrink the contiguous pass
-    c -= 4 * k - s;
-    s = 4 * k;
-  }
-  if (tp.contig_log && tune_params_valid(log_n, tp)) {
-    c = tp.contig_log;
-    s = log_n - c;
-    k = tp.num_strided;
-  }
-  uint32_t N = log_n;
-  for (uint32_t i = 0; i < k; ++i) {
-    const uint32_t part = s / k + (i < s % k ? 1 : 0);
-    pl.strided[i] = part;
-    pl.log_N[i] = N;
-    N -= part;
-  }
-  pl.num_strided = (int)k;
-  pl.contig = c;
-}
-
 // log_n < 8: one pass of the generic LDS kernel.
 int run_general(bc_context *ctx, const uint64_t *inputs, uint64_t *outputs, uint32_t log_n, uint32_t num_ntts,
                 uint32_t stride_in, uint32_t stride_out, bool bitrev_inputs, bool inverse, uint32_t log_ext,
@@ -100,18 +62,6 @@ int run_general(bc_context *ctx, const uint64_t *inputs, uint64_t *outputs, uint
 
 }  // namespace
 
-uint32_t strided_groups(uint64_t pairs, uint64_t positions, uint32_t tuned_bpw, uint32_t *bpw_out) {
-  // about 2048 (r-range, pair group) workgroups, 1..16 pairs each, unless tuned
-  uint64_t bpw = pairs * positions / 2048;
-  bpw = bpw < 1 ? 1 : (bpw > 16 ? 16 : bpw);
-  if (tuned_bpw) bpw = tuned_bpw;
-  // grid.y is at most 65535: larger batches take more pairs per workgroup
-  const uint64_t min_bpw = (pairs + STRIDED_MAX_GROUPS - 1) / STRIDED_MAX_GROUPS;
-  if (bpw < min_bpw) bpw = min_bpw;
-  *bpw_out = (uint32_t)bpw;
-  return (uint32_t)((pairs + bpw - 1) / bpw);
-}
-
 Tables current_tables(bool &ok) {
   bc_context *ctx = current_context();
   ok = ctx != nullptr;
@@ -120,70 +70,6 @@ Tables current_tables(bool &ok) {
 
 }  // namespace bc
 
-namespace bc {
-
-int describe_fast_passes(const Tables &tab, const uint64_t *inputs, uint64_t *outputs, uint32_t log_n,
-                         uint32_t num_cols, uint64_t stride_in, uint64_t stride_out, bool bitrev_inputs, bool inverse,
-                         uint32_t log_ext, uint32_t coset_index, PassDesc *out) {
-  if (log_n < 8) return -1;
-  FastPlan plan;
-  make_fast_plan(log_n, tuned(log_n, num_cols), plan);
-  const uint64_t n = 1ull << log_n;
-  FastArgs a{};
-  a.tab = tab;
-  a.num_cols = num_cols;
-  a.log_n = log_n;
-  a.log_ext = log_ext;
-  a.coset_exp = log_ext ? (uint32_t)((uint64_t)coset_index << (32 - log_n - log_ext)) : 0u;
-  a.n_inv = gl::inv(n % gl::P);
-  const int np = plan.num_strided + 1;
-  const bool single = np == 1;
-  for (int k = 0; k < np; ++k) {
-    // n2b runs strided passes top-down then the contiguous pass; b2n the reverse
-    const int p = bitrev_inputs ? (np - 1 - k) : k;  // index in n2b order
-    const bool first = k == 0, last = k == np - 1;
-    PassDesc &pd = out[k];
-    pd.a = a;
-    pd.a.in = first ? inputs : outputs;
-    pd.a.stride_in = first ? stride_in : stride_out;
-    pd.a.out = outputs;
-    pd.a.stride_out = stride_out;
-    pd.a.first = first;
-    pd.a.last = last;
-    pd.positions = pd.pairs = 0;
-    if (p == np - 1) {
-      int mode = MODE_NONE;
-      if (!inverse && log_ext && first) mode = MODE_LDE_IN;
-      else if (inverse && last && log_ext) mode = MODE_LDE_OUT;
-      else if (inverse && last && single) mode = MODE_NINV;
-      pd.contig = true;
-      pd.log_h = plan.contig;
-      pd.mode = mode;
-      pd.a.log_N = plan.contig;
-      const uint64_t tiles = log_n >= TILE_LOG ? ((uint64_t)num_cols << (log_n - TILE_LOG))
-                                               : (num_cols + (1ull << (TILE_LOG - log_n)) - 1) >> (TILE_LOG - log_n);
-      if (tiles > 0xFFFFFFFFull) return -1;
-      pd.a.tiles = (uint32_t)tiles;
-    } else {
-      const bool top = p == 0;
-      int mode = MODE_NONE;
-      if (top) {
-        if (!inverse && log_ext && !bitrev_inputs) mode = MODE_LDE_IN;
-        else if (inverse && log_ext && bitrev_inputs) mode = MODE_LDE_OUT;
-        else if (inverse && !log_ext) mode = MODE_NINV;
-      }
-      pd.contig = false;
-      pd.log_h = plan.strided[p];
-      pd.mode = mode;
-      pd.a.log_N = plan.log_N[p];
-      pd.positions = 1ull << (pd.a.log_N - TILE_LOG);
-      pd.pairs = (uint64_t)num_cols << (log_n - pd.a.log_N);
-    }
-  }
-  return np;
-}
-
-}  // namespace bc
 
 extern "C" uint32_t bc_omega_log_order(void) { return bc::OMEGA_LOG_ORDER; }
 

@@ -20,6 +20,12 @@ struct RaggedSeg {
   uint64_t col_off;     // first entry of this segment's column table (a.cols is set when packed)
 };
 
+// variant = kind << 8 | h << 4 | mc << 2 | dm, dm: 0 fwd DIF, 1 fwd DIF + coset
+// scaling of the inputs, 2 inv DIT, 3 inv DIT + n^-1
+__host__ __device__ constexpr uint32_t variant_id(bool strided, uint32_t h, bool mc, uint32_t dm) {
+  return ((strided ? 1u : 0u) << 8) | (h << 4) | ((mc ? 1u : 0u) << 2) | dm;
+}
+
 // Dependency levels of a plan: pass k of a job's first stage is level k; the
 // coset stages of an LDE follow the inverse stage's last level.
 constexpr uint32_t RAGGED_MAX_LEVELS = 16;
