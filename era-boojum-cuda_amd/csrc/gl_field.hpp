@@ -8,8 +8,9 @@
 // non-canonical add needs), mul and mul_pow2 accept any u64 and return
 // canonical values.  The implementation is derived from the two identities
 //   2^64 = 2^32 - 1 (mod p)      2^96 = -1 (mod p)
-// and is written for AMD's 32-bit VALU: 64x64 products are four
-// v_mad_u64_u32 steps, carries are read back from the add/sub carry-out.
+// and is written for AMD's 32-bit VALU: products are v_mad_u64_u32 chains
+// (whose 64-bit addend also applies 2^64 = 2^32 - 1), and carries are read
+// back from the add/sub and multiply-add carry-outs.
 #pragma once
 #include <stdint.h>
 
@@ -77,21 +78,122 @@ GL_FN uint64_t reduce_limbs(uint32_t l0, uint32_t l1, uint32_t l2, uint32_t l3) 
   return fold(((uint64_t)hi << 32) | lo, c && !b1, b1 && !c);
 }
 
-// a * b mod p for any u64 a, b; result canonical.  The 128-bit product is
-// accumulated column by column through the 64-bit addend of four
-// 32x32+64 multiply-adds; every partial sum stays below 2^64
-// ((2^32 - 1)^2 + 2 (2^32 - 1) = 2^64 - 1), so no carry needs checking:
-//   t1 = a0 b1 + hi(a0 b0),  t2 = a1 b0 + lo(t1),  t3 = a1 b1 + hi(t1),
-//   t4 = t3 + hi(t2);  limbs lo(a0 b0), lo(t2), lo(t4), hi(t4).
+#if defined(__HIP_DEVICE_COMPILE__)
+// gfx950 building blocks whose carries stay in lane masks (SGPR pairs).  A
+// VALU instruction that reads a lane mask written by another VALU
+// instruction needs two wait states; every block below that consumes such a
+// mask as a carry-in starts with s_nop 1, and masks read by SALU first need
+// none.  The blocks hold no memory operations.
+namespace dev {
+
+// a * b + c as 65 bits: the low 64 bits, and bit 64 in `carry`
+// (v_mad_u64_u32 writes the carry out of its 64-bit addition to an SGPR pair).
+__device__ __forceinline__ uint64_t mad_carry(uint32_t a, uint32_t b, uint64_t c, uint64_t &carry) {
+  uint64_t r;
+  asm("v_mad_u64_u32 %0, %1, %2, %3, %4" : "=v"(r), "=s"(carry) : "v"(a), "v"(b), "v"(c));
+  return r;
+}
+
+// z - h - cin (cin a lane mask) with the final borrow in `borrow`.
+__device__ __forceinline__ uint64_t sub32_borrow(uint64_t z, uint32_t h, uint64_t cin, uint64_t &borrow) {
+  uint32_t lo, hi;
+  uint64_t b0;
+  asm("s_nop 1\n\t"
+      "v_subb_co_u32_e64 %[lo], %[b0], %[zlo], %[h], %[cin]\n\t"
+      "s_nop 1\n\t"
+      "v_subb_co_u32_e64 %[hi], %[bo], %[zhi], 0, %[b0]"
+      : [lo] "=&v"(lo), [hi] "=&v"(hi), [b0] "=&s"(b0), [bo] "=&s"(borrow)
+      : [zlo] "v"((uint32_t)z), [zhi] "v"((uint32_t)(z >> 32)), [h] "v"(h), [cin] "s"(cin));
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// z - h with the borrow in `borrow`.
+__device__ __forceinline__ uint64_t sub32(uint64_t z, uint32_t h, uint64_t &borrow) {
+  uint32_t lo, hi;
+  uint64_t b0;
+  asm("v_sub_co_u32_e64 %[lo], %[b0], %[zlo], %[h]\n\t"
+      "s_nop 1\n\t"
+      "v_subb_co_u32_e64 %[hi], %[bo], %[zhi], 0, %[b0]"
+      : [lo] "=&v"(lo), [hi] "=&v"(hi), [b0] "=&s"(b0), [bo] "=&s"(borrow)
+      : [zlo] "v"((uint32_t)z), [zhi] "v"((uint32_t)(z >> 32)), [h] "v"(h));
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// Canonical value of x + (plus - minus) 2^64 for lane masks plus, minus,
+// given the bounds of fold() below: a set plus alone adds EPS, a set minus
+// alone adds P (subtracts EPS), and otherwise x >= P adds EPS.
+__device__ __forceinline__ uint64_t fold_masks(uint64_t x, uint64_t plus, uint64_t minus) {
+  uint32_t lo, hi;
+  uint64_t ge, pl, mi;
+  asm("v_cmp_le_u64_e64 %[ge], %[P], %[x]\n\t"
+      "s_andn2_b64 %[pl], %[cp], %[cm]\n\t"
+      "s_andn2_b64 %[mi], %[cm], %[cp]\n\t"
+      "s_andn2_b64 %[ge], %[ge], %[mi]\n\t"
+      "s_or_b64 %[ge], %[ge], %[pl]\n\t"
+      "v_cndmask_b32_e64 %[lo], 0, 1, %[mi]\n\t"
+      "v_cndmask_b32_e64 %[hi], 0, -1, %[mi]\n\t"
+      "v_cndmask_b32_e64 %[lo], %[lo], -1, %[ge]"
+      : [lo] "=&v"(lo), [hi] "=&v"(hi), [ge] "=&s"(ge), [pl] "=&s"(pl), [mi] "=&s"(mi)
+      : [x] "v"(x), [P] "s"(P), [cp] "s"(plus), [cm] "s"(minus)
+      : "scc");
+  return x + (((uint64_t)hi << 32) | lo);
+}
+
+// Canonical value of x + plus 2^64 (plus a lane mask), given x + EPS < p
+// wherever plus is set.
+__device__ __forceinline__ uint64_t fold_plus(uint64_t x, uint64_t plus) {
+  uint32_t lo;
+  uint64_t ge;
+  asm("v_cmp_le_u64_e64 %[ge], %[P], %[x]\n\t"
+      "s_or_b64 %[ge], %[ge], %[cp]\n\t"
+      "v_cndmask_b32_e64 %[lo], 0, -1, %[ge]"
+      : [lo] "=&v"(lo), [ge] "=&s"(ge)
+      : [x] "v"(x), [P] "s"(P), [cp] "s"(plus)
+      : "scc");
+  return x + (uint64_t)lo;
+}
+
+}  // namespace dev
+#endif
+
+// a * b mod p for any u64 a, b; result canonical.  Five 32x32+64
+// multiply-adds, the last one applying 2^64 = EPS:
+//   t0 = a0 b0,  t1 = a0 b1 + hi(t0)            (< 2^64)
+//   t2 = a1 b0 + t1 = T2 + c 2^64               (c: carry out, 65-bit sum)
+//   t3 = a1 b1 + hi(T2)                         (< 2^64)
+// so a b = X + lo(t3) 2^64 + (hi(t3) + c) 2^96 with X = lo(t0) + lo(T2) 2^32,
+// and with 2^64 = EPS, 2^96 = -1:
+//   a b = X + lo(t3) EPS - hi(t3) - c = Z + c2 2^64 - hi(t3) - c,
+// where Z + c2 2^64 = X + lo(t3) EPS is the fifth multiply-add.  Then
+// W = Z - hi(t3) - c with final borrow bw, and a b = W + (c2 - bw) 2^64:
+// c2 alone leaves W <= 2^64 - 2^33 (W + EPS < p); bw alone leaves
+// W >= 2^64 - 2^32 (W - EPS canonical); both or neither leave W for the
+// canonical check.  On gfx950 the carries stay in lane masks.
 GL_FN uint64_t mul(uint64_t a, uint64_t b) {
   const uint32_t a0 = (uint32_t)a, a1 = (uint32_t)(a >> 32);
   const uint32_t b0 = (uint32_t)b, b1 = (uint32_t)(b >> 32);
   const uint64_t t0 = (uint64_t)a0 * b0;
   const uint64_t t1 = (uint64_t)a0 * b1 + (t0 >> 32);
-  const uint64_t t2 = (uint64_t)a1 * b0 + (uint32_t)t1;
-  const uint64_t t3 = (uint64_t)a1 * b1 + (t1 >> 32);
-  const uint64_t t4 = t3 + (t2 >> 32);
-  return reduce_limbs((uint32_t)t0, (uint32_t)t2, (uint32_t)t4, (uint32_t)(t4 >> 32));
+#if defined(__HIP_DEVICE_COMPILE__)
+  uint64_t c, c2, bw;
+  const uint64_t t2 = dev::mad_carry(a1, b0, t1, c);
+  const uint64_t t3 = (uint64_t)a1 * b1 + (t2 >> 32);
+  const uint64_t z = dev::mad_carry((uint32_t)t3, 0xFFFFFFFFu, (t2 << 32) | (uint32_t)t0, c2);
+  const uint64_t w = dev::sub32_borrow(z, (uint32_t)(t3 >> 32), c, bw);
+  return dev::fold_masks(w, c2, bw);
+#else
+  const uint64_t p2 = (uint64_t)a1 * b0;
+  const uint64_t t2 = p2 + t1;
+  const bool c = t2 < p2;
+  const uint64_t t3 = (uint64_t)a1 * b1 + (t2 >> 32);
+  const uint64_t x = (t2 << 32) | (uint32_t)t0;
+  const uint64_t z = (uint64_t)(uint32_t)t3 * EPS + x;
+  const bool c2 = z < x;
+  unsigned b0w, bw;
+  const uint32_t lo = __builtin_subc((uint32_t)z, (uint32_t)(t3 >> 32), (unsigned)c, &b0w);
+  const uint32_t hi = __builtin_subc((uint32_t)(z >> 32), 0u, b0w, &bw);
+  return fold(((uint64_t)hi << 32) | lo, c2 && !bw, bw && !c2);
+#endif
 }
 
 // Sum of products kept unreduced in five 32-bit limbs,
@@ -127,39 +229,50 @@ GL_FN uint64_t acc_reduce(const Acc &acc) {
 }
 
 // x * 2^E for 0 < E < 64, canonical result, any u64 x.
+//
+// Device form (gfx950), one multiply-add applying 2^64 = EPS:
+//   E < 32:  x 2^E = y + h 2^64, y = x << E (mod 2^64), h = hi(x) >> (32 - E),
+//            so x 2^E = y + h EPS = Z + c 2^64 (65-bit multiply-add), and
+//            Z + EPS < p where c is set (h < 2^31 gives Z < 2^63).
+//   E >= 32: with e = E - 32, x 2^E = L + H 2^64, L = lo(x) 2^E (mod 2^64),
+//            H = x >> (32 - e) < 2^E; H 2^64 = lo(H) EPS - hi(H), so
+//            x 2^E = L + lo(H) EPS - hi(H) = Z + c 2^64 - hi(H), folded as in mul.
 template <unsigned E> GL_FN uint64_t pow2_direct(uint64_t x) {
   static_assert(E > 0 && E < 64, "");
   const uint32_t x0 = (uint32_t)x, x1 = (uint32_t)(x >> 32);
+#if defined(__HIP_DEVICE_COMPILE__)
+  uint64_t c;
   if constexpr (E < 32) {
-    // x << E = y0 + y1 2^32 + y2 2^64 = (y1 + y2) 2^32 + y0 - y2, y2 < 2^E
-    const uint32_t y0 = x0 << E;
-    const uint32_t y1 = (x1 << E) | (x0 >> (32 - E));
-    const uint32_t y2 = x1 >> (32 - E);
-    unsigned c, b0, b1;
-    uint32_t hi = __builtin_addc(y1, y2, 0u, &c);
-    const uint32_t lo = __builtin_subc(y0, y2, 0u, &b0);
-    hi = __builtin_subc(hi, 0u, b0, &b1);
-    // value = c 2^64 + (hi:lo) - b1 2^64; with c set, hi < y2 < 2^E keeps
-    // (hi:lo) + EPS below p; c = 0, b1 = 1 cannot happen
-    return fold(((uint64_t)hi << 32) | lo, c && !b1, false);
-  } else if constexpr (E == 32) {
-    return reduce_limbs(0u, x0, x1, 0u);  // x 2^32 = x0 2^32 + x1 2^64
+    const uint64_t z = dev::mad_carry(x1 >> (32 - E), 0xFFFFFFFFu, x << E, c);
+    return dev::fold_plus(z, c);
   } else {
-    // 2^E = -2^-K with K = 96 - E in (32, 64): split x = q 2^K + r, then
-    // x 2^E = r 2^E - q with r 2^E < 2^96:  r 2^E = (Lhi 2^32) + H 2^64,
-    // H 2^64 = H 2^32 - H, so x 2^E = (Lhi + H) 2^32 - (H + q).
-    constexpr unsigned K = 96 - E;
-    const uint32_t lhi = x0 << (E - 32);
-    const uint32_t rhi = x1 & ((1u << (K - 32)) - 1u);
-    const uint32_t h = (uint32_t)((((uint64_t)rhi << 32) | x0) >> (K - 32));
-    const uint32_t q = x1 >> (K - 32);
-    unsigned c, sc, b0, b1;
-    uint32_t hi = __builtin_addc(lhi, h, 0u, &c);
-    const uint32_t ssum = __builtin_addc(h, q, 0u, &sc);
-    const uint32_t lo = __builtin_subc(0u, ssum, 0u, &b0);
-    hi = __builtin_subc(hi, sc, b0, &b1);
-    return fold(((uint64_t)hi << 32) | lo, c && !b1, b1 && !c);
+    constexpr unsigned e = E - 32;
+    const uint64_t lo_part = (uint64_t)(x0 << e) << 32;
+    if constexpr (e == 0) {
+      return dev::fold_plus(dev::mad_carry(x1, 0xFFFFFFFFu, lo_part, c), c);
+    } else {
+      const uint64_t z = dev::mad_carry((uint32_t)(x >> (32 - e)), 0xFFFFFFFFu, lo_part, c);
+      uint64_t bw;
+      const uint64_t w = dev::sub32(z, x1 >> (32 - e), bw);
+      return dev::fold_masks(w, c, bw);
+    }
   }
+#else
+  if constexpr (E < 32) {
+    const uint64_t y = x << E;
+    const uint64_t z = (uint64_t)(x1 >> (32 - E)) * EPS + y;
+    return fold(z, z < y, false);
+  } else {
+    constexpr unsigned e = E - 32;
+    const uint64_t lo_part = (uint64_t)(x0 << e) << 32;
+    const uint32_t h0 = e ? (uint32_t)(x >> ((32 - e) % 32)) : x1;
+    const uint32_t h1 = e ? x1 >> ((32 - e) % 32) : 0u;
+    const uint64_t z = (uint64_t)h0 * EPS + lo_part;
+    const bool c = z < lo_part;
+    const bool bw = z < h1;
+    return fold(z - h1, c && !bw, bw && !c);
+  }
+#endif
 }
 
 // x * 2^-K for 0 < K <= 32, canonical result, any u64 x:

@@ -163,13 +163,9 @@ __device__ __forceinline__ void contig_run(const FastArgs &a, PassLds &sh, uint3
   uint64_t *tw1 = sh.tw;
   const uint32_t t = threadIdx.x;
 
-  // level-0 twists: 15 per thread, in registers for the whole kernel
-  uint64_t tw0[15];
-  {
-    const uint64_t *src = lvl_table<LH, INV>(a.tab) + (t & ((1u << (LH - 4)) - 1u));
-#pragma unroll
-    for (int i = 1; i < 16; ++i) tw0[i - 1] = src[(i - 1) << (LH - 4)];
-  }
+  // level-0 twists: 15 per thread, read from the (cache-resident) level table
+  // at each use, which keeps 30 VGPRs free
+  const uint64_t *tw0 = lvl_table<LH, INV>(a.tab) + (t & ((1u << (LH - 4)) - 1u));
   // level-1 twists (at most 240 entries) in LDS
   if constexpr (NL == 3) stage_level_twists<LH - 4, INV>(tw1, a.tab);
   const uint32_t rp1 = t & ((1u << (LH - 8)) - 1u);
@@ -217,7 +213,7 @@ __device__ __forceinline__ void contig_run(const FastArgs &a, PassLds &sh, uint3
     if constexpr (!DIT) {
       dft_dif<4, INV>(x);
 #pragma unroll
-      for (int i = 1; i < 16; ++i) x[i] = gl::mul(x[i], tw0[i - 1]);
+      for (int i = 1; i < 16; ++i) x[i] = gl::mul(x[i], tw0[(i - 1) << (LH - 4)]);
 #pragma unroll
       for (int i = 0; i < 16; ++i) lds[pidx(celem<LH, 0, 4>(t, 0, i))] = x[i];
       __syncthreads();
@@ -281,7 +277,7 @@ __device__ __forceinline__ void contig_run(const FastArgs &a, PassLds &sh, uint3
 #pragma unroll
       for (int i = 0; i < 16; ++i) x[i] = lds[pidx(celem<LH, 0, 4>(t, 0, i))];
 #pragma unroll
-      for (int i = 1; i < 16; ++i) x[i] = gl::mul(x[i], tw0[i - 1]);
+      for (int i = 1; i < 16; ++i) x[i] = gl::mul(x[i], tw0[(i - 1) << (LH - 4)]);
       dft_dit<4, INV>(x);
 #pragma unroll
       for (int i = 0; i < 16; ++i) {
