@@ -90,3 +90,11 @@ def run_plain(jobs, stream=None):
     import torch
     s = (stream or torch.cuda.current_stream()).cuda_stream
     _check(lib().bc_ragged_run_plain(_array(jobs), len(jobs), s), "ragged_run_plain")
+
+
+def run_no_alloc(jobs, stream=None):
+    """Plans and runs the jobs with the plan in the library's static device
+    array (no device allocation; results as RaggedPlan(jobs).run())."""
+    import torch
+    s = (stream or torch.cuda.current_stream()).cuda_stream
+    _check(lib().bc_ragged_run_no_alloc(_array(jobs), len(jobs), s), "ragged_run_no_alloc")

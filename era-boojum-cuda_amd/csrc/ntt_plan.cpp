@@ -8,6 +8,8 @@
 #include "ntt_internal.hpp"
 #include "ntt_ragged.hpp"
 
+#include <string.h>
+
 #include <algorithm>
 #include <map>
 #include <vector>
@@ -318,6 +320,26 @@ int ragged_build(const bc_ragged_job *jobs, uint32_t num_jobs, const Tables &tab
     out.chunk_seg[c] = sgi;
   }
   return BC_SUCCESS;
+}
+
+// Packed plan table: segments, chunk index, column tables (16-byte aligned).
+size_t ragged_chunk_offset(const RaggedHostPlan &hp) { return std::max<size_t>(1, hp.segs.size()) * sizeof(RaggedSeg); }
+static size_t cols_offset(const RaggedHostPlan &hp) {
+  return (ragged_chunk_offset(hp) + hp.chunk_seg.size() * sizeof(uint32_t) + 15) / 16 * 16;
+}
+
+size_t ragged_table_bytes(const RaggedHostPlan &hp) { return cols_offset(hp) + hp.cols.size() * sizeof(ColPtrs); }
+
+void ragged_pack_table(const RaggedHostPlan &hp, void *host_buf, const void *device_buf) {
+  uint8_t *b = (uint8_t *)host_buf;
+  const ColPtrs *dev_cols = (const ColPtrs *)((const uint8_t *)device_buf + cols_offset(hp));
+  RaggedSeg *segs = (RaggedSeg *)b;
+  for (size_t i = 0; i < hp.segs.size(); ++i) {
+    segs[i] = hp.segs[i];
+    segs[i].a.cols = dev_cols + hp.segs[i].col_off;
+  }
+  memcpy(b + ragged_chunk_offset(hp), hp.chunk_seg.data(), hp.chunk_seg.size() * sizeof(uint32_t));
+  if (!hp.cols.empty()) memcpy(b + cols_offset(hp), hp.cols.data(), hp.cols.size() * sizeof(ColPtrs));
 }
 
 }  // namespace bc

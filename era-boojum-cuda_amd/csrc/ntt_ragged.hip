@@ -25,6 +25,7 @@
 
 #include <algorithm>
 #include <map>
+#include <mutex>
 #include <vector>
 
 namespace bc {
@@ -166,28 +167,6 @@ __global__ __launch_bounds__(NT, 2) void ntt_ragged_kernel(const RaggedSeg *__re
 
 Tables current_tables(bool &ok);  // ntt_api.hip
 
-namespace {
-// byte offsets of the packed table: segments, chunk index, column tables
-size_t chunk_offset(const RaggedHostPlan &hp) { return std::max<size_t>(1, hp.segs.size()) * sizeof(RaggedSeg); }
-size_t cols_offset(const RaggedHostPlan &hp) {
-  return (chunk_offset(hp) + hp.chunk_seg.size() * sizeof(uint32_t) + 15) / 16 * 16;
-}
-}  // namespace
-
-size_t ragged_table_bytes(const RaggedHostPlan &hp) { return cols_offset(hp) + hp.cols.size() * sizeof(ColPtrs); }
-
-void ragged_pack_table(const RaggedHostPlan &hp, void *host_buf, const void *device_buf) {
-  uint8_t *b = (uint8_t *)host_buf;
-  const ColPtrs *dev_cols = (const ColPtrs *)((const uint8_t *)device_buf + cols_offset(hp));
-  RaggedSeg *segs = (RaggedSeg *)b;
-  for (size_t i = 0; i < hp.segs.size(); ++i) {
-    segs[i] = hp.segs[i];
-    segs[i].a.cols = dev_cols + hp.segs[i].col_off;
-  }
-  memcpy(b + chunk_offset(hp), hp.chunk_seg.data(), hp.chunk_seg.size() * sizeof(uint32_t));
-  if (!hp.cols.empty()) memcpy(b + cols_offset(hp), hp.cols.data(), hp.cols.size() * sizeof(ColPtrs));
-}
-
 size_t ragged_state_bytes() { return sizeof(QueueState); }
 
 uint32_t ragged_workgroups(int dev) {
@@ -202,7 +181,7 @@ hipError_t ragged_launch(const void *d_table, const RaggedHostPlan &hp, void *d_
                          hipStream_t s) {
   QueueState *st = (QueueState *)d_state;
   const RaggedSeg *d_segs = (const RaggedSeg *)d_table;
-  const uint32_t *d_chunks = (const uint32_t *)((const uint8_t *)d_table + chunk_offset(hp));
+  const uint32_t *d_chunks = (const uint32_t *)((const uint8_t *)d_table + ragged_chunk_offset(hp));
   ragged_reset_kernel<<<dim3(1), dim3(64), 0, s>>>(st);
   hipError_t err = hipGetLastError();
   for (uint32_t l = 0; l < hp.levels && err == hipSuccess; ++l) {
@@ -305,4 +284,77 @@ extern "C" int bc_ragged_run_plain(const bc_ragged_job *jobs, uint32_t num_jobs,
     if (rc != BC_SUCCESS) return rc;
   }
   return BC_SUCCESS;
+}
+
+// ---- Allocation-free variant (bc_ragged_run_no_alloc) ----
+//
+// The packed plan table and the queue state live in one file-scope device
+// array instead of hipMalloc'd buffers.  Its host-side name only stands in for
+// the symbol: kernels and copies use the per-device address that
+// hipGetSymbolAddress returns (each GPU holds its own copy).
+namespace bc {
+namespace {
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)8 << 20;
+constexpr int NOALLOC_MAX_DEVICES = 64;
+}  // namespace
+}  // namespace bc
+
+static __device__ __align__(256) uint8_t scratch[bc::NOALLOC_SCRATCH_BYTES];
+
+namespace {
+std::mutex g_noalloc_mu;
+uint8_t *g_noalloc_ptr[bc::NOALLOC_MAX_DEVICES] = {};
+uint32_t g_noalloc_wgs[bc::NOALLOC_MAX_DEVICES] = {};
+
+// Device address of `scratch` on the current device `dev` (looked up once per
+// device, with that device current) and the persistent grid size there.
+int noalloc_scratch(int dev, uint8_t **ptr, uint32_t *wgs) {
+  if (dev < 0 || dev >= bc::NOALLOC_MAX_DEVICES) return BC_ERROR_INVALID_VALUE;
+  std::lock_guard<std::mutex> lk(g_noalloc_mu);
+  if (!g_noalloc_ptr[dev]) {
+    void *p = nullptr;
+    const hipError_t err = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (err != hipSuccess) return err;
+    if (!p) return BC_ERROR_NOT_INITIALIZED;
+    g_noalloc_wgs[dev] = bc::ragged_workgroups(dev);
+    g_noalloc_ptr[dev] = (uint8_t *)p;
+  }
+  *ptr = g_noalloc_ptr[dev];
+  *wgs = g_noalloc_wgs[dev];
+  return BC_SUCCESS;
+}
+}  // namespace
+
+extern "C" int bc_ragged_run_no_alloc(const bc_ragged_job *jobs, uint32_t num_jobs, void *stream) {
+  using namespace bc;
+  if (!jobs && num_jobs) return BC_ERROR_INVALID_VALUE;
+  for (uint32_t j = 0; j < num_jobs; ++j) {
+    const int rc = ragged_validate_job(jobs[j]);
+    if (rc != BC_SUCCESS) return rc;
+  }
+  bool ok = false;
+  const Tables tab = current_tables(ok);
+  if (!ok) return BC_ERROR_NOT_INITIALIZED;
+  RaggedHostPlan hp;
+  int rc = ragged_build(jobs, num_jobs, tab, hp);
+  if (rc != BC_SUCCESS) return rc;
+  // table, then the queue state on the next 256-byte boundary
+  const size_t table_bytes = ragged_table_bytes(hp);
+  const size_t state_off = (table_bytes + 255) & ~(size_t)255;
+  if (state_off + ragged_state_bytes() > NOALLOC_SCRATCH_BYTES) return BC_ERROR_OUT_OF_MEMORY;
+  int dev = 0;
+  hipError_t err = hipGetDevice(&dev);
+  if (err != hipSuccess) return err;
+  uint8_t *base = nullptr;
+  uint32_t wgs = 0;
+  rc = noalloc_scratch(dev, &base, &wgs);
+  if (rc != BC_SUCCESS) return rc;
+  // The staging copy is pageable host memory, so HIP performs the upload
+  // synchronously: the vector may be released when the call returns.
+  std::vector<uint64_t> packed((table_bytes + 7) / 8);
+  ragged_pack_table(hp, packed.data(), base);
+  const hipStream_t s = (hipStream_t)stream;
+  if (table_bytes) err = hipMemcpyAsync(base, packed.data(), table_bytes, hipMemcpyHostToDevice, s);
+  if (err != hipSuccess) return err;
+  return ragged_launch(base, hp, base + state_off, wgs, s);
 }

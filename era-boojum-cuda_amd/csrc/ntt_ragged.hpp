@@ -56,6 +56,8 @@ uint32_t ragged_workgroups(int dev);
 // address (device_buf) as well as the host staging copy.
 size_t ragged_table_bytes(const RaggedHostPlan &hp);
 void ragged_pack_table(const RaggedHostPlan &hp, void *host_buf, const void *device_buf);
+// Byte offset of the chunk index inside the packed table.
+size_t ragged_chunk_offset(const RaggedHostPlan &hp);
 // Resets the queue state and launches the persistent kernel once per
 // dependency level on `s`; the packed table must already be in device memory
 // at d_table.  Allocates nothing.

@@ -34,6 +34,7 @@ extern "C" {
 
 #define BC_SUCCESS 0
 #define BC_ERROR_INVALID_VALUE 1   /* == hipErrorInvalidValue */
+#define BC_ERROR_OUT_OF_MEMORY 2   /* == hipErrorOutOfMemory */
 #define BC_ERROR_NOT_INITIALIZED 3 /* == hipErrorNotInitialized: no context */
 
 typedef struct bc_context bc_context;
@@ -151,6 +152,20 @@ int bc_ragged_plan_info(const bc_ragged_plan *plan, uint64_t *units, uint32_t *l
 /* The plain path for comparison: the same jobs as separate launch sequences
  * (one bc_batch_ntt_internal / bc_batch_lde per job). */
 int bc_ragged_run_plain(const bc_ragged_job *jobs, uint32_t num_jobs, void *stream);
+/* Plans and runs the jobs in one call without allocating device memory: no
+ * hipMalloc/hipFree, no allocator of any library.  No counterpart in the
+ * reference; it is for callers that cannot allocate (tight serving loops,
+ * harnesses that forbid it).  The packed plan table and the queue state go to
+ * a static 8 MiB device array that every GPU running the library keeps until
+ * the process ends; a plan that needs more returns BC_ERROR_OUT_OF_MEMORY
+ * before anything is launched (the 1000-job ragged benchmark batch takes
+ * about 0.12 MiB).  The table is uploaded from pageable host memory, which HIP
+ * copies synchronously, then the levels are launched on `stream` exactly as
+ * bc_ragged_run does, with the same results.  Every call on one GPU shares
+ * that GPU's array and overwrites it: calls that overlap on one GPU (other
+ * threads, or work still queued on another stream) race.  Host planning uses
+ * the process heap. */
+int bc_ragged_run_no_alloc(const bc_ragged_job *jobs, uint32_t num_jobs, void *stream);
 
 /* ---- Order conversion and domain powers around the NTT ---- */
 

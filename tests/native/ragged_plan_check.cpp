@@ -169,6 +169,18 @@ int main() {
   bad.stride_in = (1ull << bad.log_n) - 1;
   REQUIRE(ragged_build(&bad, 1, tab, hw) == BC_ERROR_INVALID_VALUE);
 
-  printf("OK %zu %u %llu\n", hp.segs.size(), hp.levels, (unsigned long long)hp.total_items);
+  // packed device image: pack into a host buffer (under the sanitizers) and
+  // check every segment's column table lands inside it
+  const size_t table_bytes = ragged_table_bytes(hp);
+  std::vector<uint64_t> packed((table_bytes + 7) / 8 + 1, 0);
+  const uint64_t fake_base = 0x100000000ull;
+  ragged_pack_table(hp, packed.data(), (const void *)(uintptr_t)fake_base);
+  const RaggedSeg *ps = (const RaggedSeg *)packed.data();
+  for (size_t i = 0; i < hp.segs.size(); ++i) {
+    const uint64_t col_addr = (uint64_t)(uintptr_t)ps[i].a.cols;
+    REQUIRE(col_addr >= fake_base + ragged_chunk_offset(hp));
+    REQUIRE(col_addr + (uint64_t)ps[i].a.num_cols * sizeof(ColPtrs) <= fake_base + table_bytes);
+  }
+  printf("OK %zu %u %llu %zu\n", hp.segs.size(), hp.levels, (unsigned long long)hp.total_items, table_bytes);
   return 0;
 }

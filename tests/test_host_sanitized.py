@@ -33,10 +33,13 @@ def test_ragged_planner_invariants_sanitized(tmp_path):
                    ["-o", str(exe), os.path.join(HERE, "native", "ragged_plan_check.cpp"),
                     os.path.join(CSRC, "ntt_plan.cpp"), os.path.join(CSRC, "ntt_tune.cpp"), "-ldl"], check=True)
     words = run(exe).split()
-    segments, levels, items = int(words[1]), int(words[2]), int(words[3])
+    segments, levels, items, table_bytes = int(words[1]), int(words[2]), int(words[3]), int(words[4])
     # 1000 jobs of 14 sizes and 4 operations: several segments per level, at
     # most 3 passes per inverse stage plus 3 per coset stage
     assert segments > 100 and 2 <= levels <= 6 and items > segments
+    # the packed table fits the 8 MiB static array of bc_ragged_run_no_alloc
+    # with a wide margin
+    assert table_bytes < (1 << 20)
 
 
 @pytest.mark.skipif(shutil.which("gcc") is None, reason="gcc not available")
