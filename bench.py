@@ -414,6 +414,9 @@ def main():
     ap.add_argument("--cols", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-cols", type=int, default=256)
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="threads of the CPU baseline (default: every core this process may run on, at most "
+                         "the 16-core host share of one GPU)")
     ap.add_argument("--no-lde", action="store_true", help="skip the coset-LDE measurement")
     ap.add_argument("--no-ragged", action="store_true", help="skip the ragged-batch measurement")
     ap.add_argument("--ragged-jobs", type=int, default=1000)
@@ -554,9 +557,10 @@ def main():
     }
     rec.update(extras)
     if not args.no_cpu:
-        # the host cores this process may run on, at most the 16 a one-GPU
-        # share of the host provides
-        threads = min(CPU_SHARE, len(os.sched_getaffinity(0)))
+        # every host core this process may run on, at most the 16 a one-GPU
+        # share of the host provides (on a one-GPU box that is all of them);
+        # --cpu-threads overrides it on a larger host
+        threads = args.cpu_threads or min(CPU_SHARE, len(os.sched_getaffinity(0)))
         progress(f"cpu baseline on {threads} threads")
         rec["cpu_baseline"] = cpu_baseline(log_n, args.cpu_cols, threads)
     print(json.dumps(rec), flush=True)

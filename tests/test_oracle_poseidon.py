@@ -100,12 +100,17 @@ def test_dense_row0_is_mds_row0(grain):
     assert [int(x) for x in grain[1]["dense"][0]] == [1 << e for e in EXPS]
 
 
-def test_constant_dependent_tables_are_not_claimed(grain):
-    """Grain constants are not boojum's: the derived constant tables differ
-    from the reference's, which is why digests stay parity-unpinned."""
+@pytest.mark.skip(reason="parity unpinned: boojum's 360 Poseidon round constants are not in the reference "
+                         "tree, and no Grain LFSR seeding tried (field 0/1, S-box 0/1, field size 63/64 bits, "
+                         "big- or little-endian 64-bit draws) reproduces the reference's constant-dependent "
+                         "fused tables (native/poseidon_constants_template.cuh:48,65)")
+def test_constant_dependent_tables_match_reference(grain):
+    """The two constant-dependent fused tables derived from the candidate
+    constants equal the reference's (checked through the fixture's SHA-256).
+    It would pin the digests once boojum's constants are available."""
     for key, name in (("rcf", "ROUND_CONSTANTS_FUSED_LAST_FULL_AND_FIRST_PARTIAL"),
                       ("sbox_c", "ROUND_CONSTANTS_FOR_FUSED_S_BOXES")):
-        assert sha(grain[1][key]) != GOLDEN["tables"][name]["sha256"]
+        assert sha(grain[1][key]) == GOLDEN["tables"][name]["sha256"]
 
 
 # ---- permutation ------------------------------------------------------------
