@@ -71,7 +71,7 @@ def bench_lde(log_n, cols, log_blowup, reps, warmup):
                          "algorithmic_bytes_per_call": algo}}
 
 
-def bench_ragged(count, log_lo, log_hi, reps, warmup):
+def bench_ragged(count, log_lo, log_hi, reps, warmup, plain=True):
     """Ragged batch: the persistent work-queue kernel over every job (one
     launch per dependency level) vs the plain path (separate launch sequences
     per job, on the same stream).  Both must give
@@ -104,8 +104,8 @@ def bench_ragged(count, log_lo, log_hi, reps, warmup):
 
     ms_ragged = timed(plan.run)
     ragged_out = [j.outputs.clone() for j in jobs]
-    ms_plain = timed(lambda: ragged.run_plain(jobs))
-    same = all(torch.equal(a, j.outputs) for a, j in zip(ragged_out, jobs))
+    ms_plain = timed(lambda: ragged.run_plain(jobs)) if plain else float("nan")
+    same = all(torch.equal(a, j.outputs) for a, j in zip(ragged_out, jobs)) if plain else None
     plan.destroy()
     del ragged_out, jobs
     torch.cuda.empty_cache()
@@ -116,7 +116,7 @@ def bench_ragged(count, log_lo, log_hi, reps, warmup):
             "input_elems": in_elems, "output_elems": out_elems,
             "output_elems_per_s": out_elems / (ms_ragged * 1e-3),
             "units": info["units"], "levels": info["levels"], "workgroups": info["workgroups"],
-            "bit_identical_to_plain": bool(same),
+            "bit_identical_to_plain": same if same is None else bool(same),
             "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": gbs / HBM_PEAK_GBS, "algorithmic_bytes_per_call": algo}}
 
@@ -405,6 +405,42 @@ def progress(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def other_legs(args):
+    """Every measurement leg after the batched NTT, as one dict."""
+    import torch
+    log_n, cols = args.log_n, args.cols
+    extras = {}
+    if not args.no_latency:
+        extras["latency"] = bench_latency()
+        progress(f"latency done: {extras['latency']['gpu_us_per_call']:.1f} us per 2^16 call")
+    if not args.no_lde:
+        extras["lde"] = bench_lde(20, 128, 2, max(3, args.steps // 2), 2)
+        torch.cuda.empty_cache()
+        progress(f"coset LDE done: {extras['lde']['ms_per_call']:.3f} ms per call")
+    if not args.no_ragged:
+        extras["ragged"] = bench_ragged(args.ragged_jobs, 10, 23, 5, 1, plain=not args.ragged_no_plain)
+        progress(f"ragged done: {extras['ragged']['ms_ragged']:.2f} ms vs plain {extras['ragged']['ms_plain']:.2f} ms")
+    if not args.no_ops:
+        torch.cuda.empty_cache()
+        extras["ops"] = bench_ops(log_n, cols, 5)
+        progress(f"ops done: bit_reverse {extras['ops']['bit_reverse']['GB/s']:.0f} GB/s")
+    if not args.no_bary:
+        torch.cuda.empty_cache()
+        extras["barycentric"] = bench_barycentric(log_n, cols, 5)
+        progress(f"barycentric done: base_at_ext {extras['barycentric']['base_at_ext']['GB/s']:.0f} GB/s")
+    if not args.no_poseidon:
+        torch.cuda.empty_cache()
+        extras["poseidon"] = bench_poseidon(log_n, 128, 3)
+        progress(f"poseidon done: tree {extras['poseidon']['poseidon']['ms_tree']:.2f} ms, "
+                 f"poseidon2 {extras['poseidon']['poseidon2']['ms_tree']:.2f} ms")
+    if not args.no_serve:
+        torch.cuda.empty_cache()
+        extras["serve"] = bench_serve(args.serve_requests, args.serve_rate, int(args.serve_budget_gb * 1e9))
+        progress(f"serving done: p50 {extras['serve']['p50_latency_ms']:.1f} ms, "
+                 f"p99 {extras['serve']['p99_latency_ms']:.1f} ms")
+    return extras
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -420,6 +456,8 @@ def main():
     ap.add_argument("--no-lde", action="store_true", help="skip the coset-LDE measurement")
     ap.add_argument("--no-ragged", action="store_true", help="skip the ragged-batch measurement")
     ap.add_argument("--ragged-jobs", type=int, default=1000)
+    ap.add_argument("--ragged-no-plain", action="store_true",
+                    help="skip the per-job comparison path of the ragged leg (profiling runs)")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop measurement")
     ap.add_argument("--no-ops", action="store_true", help="skip the bit-reverse / powers / omega-shift measurement")
     ap.add_argument("--no-bary", action="store_true", help="skip the barycentric-evaluation measurement")
@@ -428,6 +466,8 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=100)
     ap.add_argument("--serve-rate", type=float, default=25.0, help="offered load, requests per second")
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
+    ap.add_argument("--legs-only", action="store_true",
+                    help="skip the batched-NTT measurement and print only the other legs (for per-leg profiling)")
     ap.add_argument("--mode", choices=["ntt", "copy"], default="ntt",
                     help="copy: time a plain device copy of the same matrix (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -462,6 +502,12 @@ def main():
         ctx.destroy()
         return
     variants = [(False, False), (True, False), (False, True), (True, True)]  # (bitrev_inputs, inverse)
+    if args.legs_only:
+        del x, y
+        torch.cuda.empty_cache()
+        print(json.dumps(other_legs(args)), flush=True)
+        ctx.destroy()
+        return
 
     def step():
         for br, inv in variants:
@@ -503,37 +549,8 @@ def main():
     copy_gbs = algo_bytes / (cev[0].elapsed_time(cev[1]) / 5 * 1e-3) / 1e9
     del x, y
     progress(f"batched NTT done: {avg_call_ms:.3f} ms per call")
-    extras = {}
-    if not args.no_latency:
-        extras["latency"] = bench_latency()
-        progress(f"latency done: {extras['latency']['gpu_us_per_call']:.1f} us per 2^16 call")
-    if not args.no_lde:
-        extras["lde"] = bench_lde(20, 128, 2, max(3, args.steps // 2), 2)
-        torch.cuda.empty_cache()
-        progress(f"coset LDE done: {extras['lde']['ms_per_call']:.3f} ms per call")
-    if not args.no_ragged:
-        extras["ragged"] = bench_ragged(args.ragged_jobs, 10, 23, 5, 1)
-        progress(f"ragged done: {extras['ragged']['ms_ragged']:.2f} ms vs plain {extras['ragged']['ms_plain']:.2f} ms")
-    if not args.no_ops:
-        torch.cuda.empty_cache()
-        extras["ops"] = bench_ops(log_n, cols, 5)
-        progress(f"ops done: bit_reverse {extras['ops']['bit_reverse']['GB/s']:.0f} GB/s")
-    if not args.no_bary:
-        torch.cuda.empty_cache()
-        extras["barycentric"] = bench_barycentric(log_n, cols, 5)
-        progress(f"barycentric done: base_at_ext {extras['barycentric']['base_at_ext']['GB/s']:.0f} GB/s")
-    if not args.no_poseidon:
-        torch.cuda.empty_cache()
-        extras["poseidon"] = bench_poseidon(log_n, 128, 3)
-        progress(f"poseidon done: tree {extras['poseidon']['poseidon']['ms_tree']:.2f} ms, "
-                 f"poseidon2 {extras['poseidon']['poseidon2']['ms_tree']:.2f} ms")
-    if not args.no_serve:
-        torch.cuda.empty_cache()
-        extras["serve"] = bench_serve(args.serve_requests, args.serve_rate, int(args.serve_budget_gb * 1e9))
-        progress(f"serving done: p50 {extras['serve']['p50_latency_ms']:.1f} ms, "
-                 f"p99 {extras['serve']['p99_latency_ms']:.1f} ms")
+    extras = other_legs(args)
     ctx.destroy()
-
     rec = {
         "metric": METRIC,
         "value": value,
