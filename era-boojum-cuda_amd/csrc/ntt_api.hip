@@ -175,6 +175,9 @@ extern "C" int bc_batch_ntt_internal(const uint64_t *inputs, uint64_t *outputs, 
                        coset_exp, gl::inv(n % gl::P), s);
   }
 
+  if (small_path_applies(log_n, num_ntts, log_ext))
+    return launch_small(ctx->tab, inputs, outputs, num_ntts, stride_in, stride_out, bitrev_inputs, inverse, s);
+
   PassDesc passes[8];
   const int np = describe_fast_passes(ctx->tab, inputs, outputs, log_n, num_ntts, stride_in, stride_out,
                                       bitrev_inputs, inverse, log_ext, coset_index, passes);

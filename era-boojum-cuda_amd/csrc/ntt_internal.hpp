@@ -141,6 +141,14 @@ constexpr uint32_t TILE_LOG = 12;
 constexpr uint32_t TILE = 1u << TILE_LOG;
 constexpr uint32_t TILE_THREADS = 256;
 
+// Latency path (ntt_small.hip): 1..SMALL_MAX_COLS columns of 2^SMALL_LOG_N
+// points with no coset, as two launches of one-wave workgroups.
+constexpr uint32_t SMALL_LOG_N = 16;
+constexpr uint32_t SMALL_MAX_COLS = 4;
+bool small_path_applies(uint32_t log_n, uint32_t num_cols, uint32_t log_ext);
+hipError_t launch_small(const Tables &tab, const uint64_t *inputs, uint64_t *outputs, uint32_t num_cols,
+                        uint64_t stride_in, uint64_t stride_out, bool bitrev_inputs, bool inverse, hipStream_t s);
+
 // Host: the split of log_n into per-pass sub-transform sizes, n2b order
 // (first entry = the pass that works on the top index bits).
 int plan_passes(uint32_t log_n, uint32_t *log_h, int max_passes);

@@ -184,3 +184,27 @@ def test_full_size_round_trip_and_sampled_parity(gpu_ctx):
         for c in (0, 77, 255):
             want = oracle.ntt_column(xh[c * n:(c + 1) * n], log_n, bitrev, False)
             assert np.array_equal(yh[c * n:(c + 1) * n], want), (bitrev, c)
+
+
+# --- latency path: 1..4 columns of 2^16 (one-wave workgroups) ----------------
+
+@pytest.mark.parametrize("num_ntts", [1, 2, 4])
+@pytest.mark.parametrize("inverse", [False, True])
+def test_latency_path_2e16_matches_oracle(gpu_ctx, num_ntts, inverse):
+    correctness([16], inverse, 0, 0, num_ntts, seed=11 + num_ntts)
+
+
+def test_latency_path_equals_batched_path(gpu_ctx):
+    """4 columns take the latency path, 5 columns the 4096-point pass kernels:
+    the first four columns must come out identical, in all four variants."""
+    import torch
+    from boojum_cuda_amd import ntt
+    n = 1 << 16
+    x = to_dev(oracle.fill_raw(5 * n, 21, 0))
+    for bitrev in (False, True):
+        for inverse in (False, True):
+            y4 = torch.empty(4 * n, dtype=torch.int64, device="cuda")
+            y5 = torch.empty(5 * n, dtype=torch.int64, device="cuda")
+            ntt.batch_ntt_out_of_place(x, y4, 16, 4, 0, 0, n, n, bitrev, inverse, 0, 0)
+            ntt.batch_ntt_out_of_place(x, y5, 16, 5, 0, 0, n, n, bitrev, inverse, 0, 0)
+            assert np.array_equal(to_host(y4), to_host(y5)[:4 * n]), (bitrev, inverse)
