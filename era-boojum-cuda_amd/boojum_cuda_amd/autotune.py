@@ -1,7 +1,9 @@
 """Autotuner for the launch parameters of the batched NTT (bc_tune_* C ABI).
 
 For each (log_n, column-count bucket) it times candidate plans on the GPU:
-the split of log_n into a contiguous pass and strided passes, the strided
+the workgroup tile (for 2^16 columns: one-wave 256-point passes or the
+4096-point pass kernels), the split of log_n into a contiguous pass and
+strided passes, the strided
 (column, block) pairs per workgroup and the contiguous grid cap, by
 coordinate search from the built-in plan, then the ragged work-unit sizes.
 A candidate is timed as one forward natural->bit-reversed call plus one
@@ -25,12 +27,13 @@ from ._lib import lib
 
 MAX_TUNE_ELEMS = 1 << 28  # larger shapes reuse the nearest tuned bucket
 MIN_GAIN = 0.02           # a candidate must beat the incumbent by 2% (timing noise)
+SMALL_LOG_N, SMALL_MAX_COLS = 16, 4  # the built-in latency-path rule (csrc/ntt_internal.hpp)
 
 
 class TuneParams(ctypes.Structure):
     _fields_ = [("contig_log", ctypes.c_uint32), ("num_strided", ctypes.c_uint32), ("strided_bpw", ctypes.c_uint32),
                 ("contig_grid", ctypes.c_uint32), ("ragged_unit_contig", ctypes.c_uint32),
-                ("ragged_unit_strided", ctypes.c_uint32)]
+                ("ragged_unit_strided", ctypes.c_uint32), ("tile_log", ctypes.c_uint32)]
 
     def key(self):
         return tuple(getattr(self, f) for f, _ in self._fields_)
@@ -186,17 +189,29 @@ def tune_shape(log_n, log_cols, reps=5, log=print):
     def better(ms, incumbent):
         return ms < incumbent * (1.0 - MIN_GAIN)
 
+    # 0. workgroup tile: for 2^16 columns, the one-wave 256-point passes
+    # against the 4096-point pass kernels (whichever the built-in rule does
+    # not pick for this column count); the rest of the search applies to the
+    # 4096-point kernels only
+    if log_n == SMALL_LOG_N:
+        alt = 12 if num_cols <= SMALL_MAX_COLS else 8
+        p = TuneParams()
+        p.tile_log = alt
+        ms = trial(p)
+        if better(ms, best_ms):
+            best, best_ms = p, ms
+    small = best.tile_log == 8 or (log_n == SMALL_LOG_N and best.tile_log == 0 and num_cols <= SMALL_MAX_COLS)
     # 1. pass split (the built-in split itself is not a candidate: an entry
     # that repeats it could only record timing noise)
-    for c, k in splits(log_n):
+    for c, k in ([] if small else splits(log_n)):
         if (c, k) == builtin_split(log_n):
             continue
-        p = TuneParams(c, k, 0, 0, 0, 0)
+        p = TuneParams(c, k, 0, 0, 0, 0, 0)
         ms = trial(p)
         if better(ms, best_ms):
             best, best_ms = p, ms
     # 2. strided pairs per workgroup
-    if log_n > 12:
+    if log_n > 12 and not small:
         for bpw in (1, 2, 4, 8, 16, 32):
             p = TuneParams(*best.key())
             p.strided_bpw = bpw
@@ -205,7 +220,7 @@ def tune_shape(log_n, log_cols, reps=5, log=print):
                 best, best_ms = p, ms
     # 3. contiguous grid cap (the built-in default is two rounds of resident
     # workgroups; 768 and 2304 are one and three rounds on 256 CUs x 3)
-    for grid in (512, 768, 1024, 2304, 4096, 8192):
+    for grid in (() if small else (512, 768, 1024, 2304, 4096, 8192)):
         p = TuneParams(*best.key())
         p.contig_grid = grid
         ms = trial(p)
@@ -248,7 +263,7 @@ def tune_shape(log_n, log_cols, reps=5, log=print):
         if not better(min(u_ms), min(d_ms)):
             best = plain
     # keep the entry only if it differs from the built-in plan and gives the same bits
-    if best.key() == (0, 0, 0, 0, 0, 0):
+    if best.key() == TuneParams().key():
         set_entry(log_n, log_cols, None)
     else:
         set_entry(log_n, log_cols, best)
@@ -272,6 +287,8 @@ def main(argv=None):
     ap.add_argument("--log-cols", default="0-9")
     ap.add_argument("--reps", type=int, default=5)
     ap.add_argument("--out", required=True)
+    ap.add_argument("--merge-from", default=None,
+                    help="database whose entries for the log_n values not tuned here are kept in --out")
     args = ap.parse_args(argv)
     import torch
     from .context import Context
@@ -283,6 +300,13 @@ def main(argv=None):
         for log_cols in _range(args.log_cols):
             tune_shape(log_n, log_cols, args.reps, log=lambda m: print(m, flush=True))
         torch.cuda.empty_cache()
+    if args.merge_from:
+        tuned_ns = set(_range(args.log_n))
+        for line in open(args.merge_from):
+            f = line.split()
+            if not f or f[0].startswith("#") or int(f[0]) in tuned_ns:
+                continue
+            set_entry(int(f[0]), int(f[1]), TuneParams(*[int(v) for v in f[2:]]))
     os.makedirs(os.path.dirname(os.path.abspath(args.out)), exist_ok=True)
     save(args.out)
     print(f"saved {args.out} in {time.time() - t0:.0f} s", flush=True)

@@ -76,7 +76,8 @@ def batch_ntt_in_place(inputs_matrix, log_n, num_ntts, inputs_offset, stride_bet
 class _PassInfo(ctypes.Structure):
     _fields_ = [("contiguous", ctypes.c_uint32), ("log_h", ctypes.c_uint32), ("log_block", ctypes.c_uint32),
                 ("mode", ctypes.c_uint32), ("work_items", ctypes.c_uint64), ("positions", ctypes.c_uint64),
-                ("groups", ctypes.c_uint64), ("pairs_per_workgroup", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+                ("groups", ctypes.c_uint64), ("pairs_per_workgroup", ctypes.c_uint32),
+                ("workgroup_threads", ctypes.c_uint32)]
 
 
 def describe_plan(log_n, num_ntts, bitrev_inputs=False, inverse=False, log_extension_degree=0, coset_index=0):
@@ -87,5 +88,5 @@ def describe_plan(log_n, num_ntts, bitrev_inputs=False, inverse=False, log_exten
     rc = lib().bc_ntt_describe_plan(log_n, num_ntts, bool(bitrev_inputs), bool(inverse), log_extension_degree,
                                     coset_index, arr, 8, ctypes.byref(n))
     _check(rc, "ntt_describe_plan")
-    return [{name: getattr(arr[i], name) for name, _ in _PassInfo._fields_ if name != "reserved"}
+    return [{name: getattr(arr[i], name) for name, _ in _PassInfo._fields_}
             for i in range(n.value)]

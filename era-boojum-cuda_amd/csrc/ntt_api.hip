@@ -175,14 +175,14 @@ extern "C" int bc_batch_ntt_internal(const uint64_t *inputs, uint64_t *outputs, 
                        coset_exp, gl::inv(n % gl::P), s);
   }
 
-  if (small_path_applies(log_n, num_ntts, log_ext))
+  const TuneParams tp = tuned(log_n, num_ntts);
+  if (small_path_applies(log_n, num_ntts, log_ext, tp))
     return launch_small(ctx->tab, inputs, outputs, num_ntts, stride_in, stride_out, bitrev_inputs, inverse, s);
 
   PassDesc passes[8];
   const int np = describe_fast_passes(ctx->tab, inputs, outputs, log_n, num_ntts, stride_in, stride_out,
                                       bitrev_inputs, inverse, log_ext, coset_index, passes);
   if (np <= 0) return BC_ERROR_INVALID_VALUE;
-  const TuneParams tp = tuned(log_n, num_ntts);
   // every pass's launch shape is fixed before the first launch, so a shape
   // that cannot be launched is refused with nothing queued (an in-place
   // transform would otherwise lose its input to the passes already run)
@@ -336,6 +336,27 @@ extern "C" int bc_ntt_describe_plan(uint32_t log_n, uint32_t num_ntts, bool bitr
   if (coset_index > 0 && (log_ext == 0 || (uint64_t)coset_index >= (1ull << log_ext))) return BC_ERROR_INVALID_VALUE;
   *num_passes = 0;
   if (log_n < 8 || num_ntts == 0) return BC_SUCCESS;
+  if (small_path_applies(log_n, num_ntts, log_ext, tuned(log_n, num_ntts))) {
+    // two one-wave passes of 8 bits: the top one (stride 256, pass twist,
+    // n^-1 for inverses) and the bottom one; b2n runs them in reverse
+    if (!passes || max_passes < 2) return BC_ERROR_INVALID_VALUE;
+    for (int k = 0; k < 2; ++k) {
+      const bool top = (k == 0) != bitrev_inputs;
+      bc_pass_info &o = passes[k];
+      o = bc_pass_info{};
+      o.contiguous = top ? 0u : 1u;
+      o.log_h = 8;
+      o.log_block = top ? SMALL_LOG_N : 8;
+      o.mode = top && inverse ? (uint32_t)MODE_NINV : (uint32_t)MODE_NONE;
+      o.work_items = (uint64_t)num_ntts << 8;
+      o.positions = top ? 256 : 0;
+      o.groups = top ? num_ntts : 0;
+      o.pairs_per_workgroup = top ? 1 : 0;
+      o.workgroup_threads = 64;
+    }
+    *num_passes = 2;
+    return BC_SUCCESS;
+  }
   PassDesc pd[8];
   const int np = describe_fast_passes(Tables{}, nullptr, nullptr, log_n, num_ntts, 1ull << log_n, 1ull << log_n,
                                       bitrev_inputs, inverse, log_ext, coset_index, pd);
@@ -349,6 +370,7 @@ extern "C" int bc_ntt_describe_plan(uint32_t log_n, uint32_t num_ntts, bool bitr
     o.log_h = pd[k].log_h;
     o.log_block = pd[k].a.log_N;
     o.mode = (uint32_t)pd[k].mode;
+    o.workgroup_threads = 256;
     if (pd[k].contig) {
       o.work_items = pd[k].a.tiles;
     } else {

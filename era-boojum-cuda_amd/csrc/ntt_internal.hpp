@@ -132,6 +132,7 @@ struct TuneParams {
   uint32_t contig_grid;   // grid cap of contiguous passes
   uint32_t unit_contig;   // ragged: tiles per work unit
   uint32_t unit_strided;  // ragged: pairs per work unit
+  uint32_t tile_log;      // 8: one-wave 256-point passes (log_n 16); 12: 4096-point passes; 0: built-in
 };
 TuneParams tuned(uint32_t log_n, uint32_t num_cols);
 bool tune_params_valid(uint32_t log_n, const TuneParams &p);
@@ -145,7 +146,8 @@ constexpr uint32_t TILE_THREADS = 256;
 // points with no coset, as two launches of one-wave workgroups.
 constexpr uint32_t SMALL_LOG_N = 16;
 constexpr uint32_t SMALL_MAX_COLS = 4;
-bool small_path_applies(uint32_t log_n, uint32_t num_cols, uint32_t log_ext);
+// Whether a call of this shape takes the latency path under tuned params tp.
+bool small_path_applies(uint32_t log_n, uint32_t num_cols, uint32_t log_ext, const TuneParams &tp);
 hipError_t launch_small(const Tables &tab, const uint64_t *inputs, uint64_t *outputs, uint32_t num_cols,
                         uint64_t stride_in, uint64_t stride_out, bool bitrev_inputs, bool inverse, hipStream_t s);
 

@@ -214,8 +214,9 @@ template <bool INV, bool DIT> hipError_t launch_pair(const FastArgs &top, const 
 
 }  // namespace
 
-bool small_path_applies(uint32_t log_n, uint32_t num_cols, uint32_t log_ext) {
-  return log_n == SMALL_LOG_N && log_ext == 0 && num_cols >= 1 && num_cols <= SMALL_MAX_COLS;
+bool small_path_applies(uint32_t log_n, uint32_t num_cols, uint32_t log_ext, const TuneParams &tp) {
+  if (log_n != SMALL_LOG_N || log_ext != 0 || num_cols < 1 || num_cols > 65535) return false;
+  return tp.tile_log == 8 || (tp.tile_log == 0 && num_cols <= SMALL_MAX_COLS);
 }
 
 hipError_t launch_small(const Tables &tab, const uint64_t *inputs, uint64_t *outputs, uint32_t num_cols,

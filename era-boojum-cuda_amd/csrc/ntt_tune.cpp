@@ -44,6 +44,8 @@ bool tune_params_valid(uint32_t log_n, const TuneParams &p) {
   } else if (p.num_strided) {
     return false;
   }
+  if (p.tile_log != 0 && p.tile_log != 8 && p.tile_log != TILE_LOG) return false;
+  if (p.tile_log == 8 && log_n != SMALL_LOG_N) return false;
   return p.strided_bpw <= 64 && p.contig_grid <= 65535 && p.unit_contig <= 64 && p.unit_strided <= 64;
 }
 
@@ -76,12 +78,14 @@ int tune_load_file(const char *path) {
     const char *p = line;
     while (*p == ' ' || *p == '\t') ++p;
     if (*p == '#' || *p == '\n' || *p == '\0') continue;
-    unsigned v[8];
-    if (sscanf(p, "%u %u %u %u %u %u %u %u", &v[0], &v[1], &v[2], &v[3], &v[4], &v[5], &v[6], &v[7]) != 8) {
+    unsigned v[9] = {};
+    const int got =
+        sscanf(p, "%u %u %u %u %u %u %u %u %u", &v[0], &v[1], &v[2], &v[3], &v[4], &v[5], &v[6], &v[7], &v[8]);
+    if (got != 8 && got != 9) {
       rc = BC_ERROR_INVALID_VALUE;
       break;
     }
-    const TuneParams t{v[2], v[3], v[4], v[5], v[6], v[7]};
+    const TuneParams t{v[2], v[3], v[4], v[5], v[6], v[7], v[8]};
     if (v[0] < 1 || v[0] > OMEGA_LOG_ORDER || v[1] > 31 || !tune_params_valid(v[0], t)) {
       rc = BC_ERROR_INVALID_VALUE;
       break;
@@ -124,8 +128,8 @@ extern "C" int bc_tune_set(uint32_t log_n, uint32_t log_cols, const bc_tune_para
     g_tune.erase(tune_key(log_n, log_cols));
     return BC_SUCCESS;
   }
-  const TuneParams t{p->contig_log, p->num_strided, p->strided_bpw, p->contig_grid, p->ragged_unit_contig,
-                     p->ragged_unit_strided};
+  const TuneParams t{p->contig_log,         p->num_strided,         p->strided_bpw, p->contig_grid,
+                     p->ragged_unit_contig, p->ragged_unit_strided, p->tile_log};
   if (!tune_params_valid(log_n, t)) return BC_ERROR_INVALID_VALUE;
   g_tune[tune_key(log_n, log_cols)] = t;
   return BC_SUCCESS;
@@ -140,6 +144,7 @@ extern "C" int bc_tune_get(uint32_t log_n, uint32_t num_cols, bc_tune_params *p)
   p->contig_grid = t.contig_grid;
   p->ragged_unit_contig = t.unit_contig;
   p->ragged_unit_strided = t.unit_strided;
+  p->tile_log = t.tile_log;
   return BC_SUCCESS;
 }
 
@@ -160,13 +165,13 @@ extern "C" int bc_tune_save(const char *path) {
   FILE *f = fopen(path, "w");
   if (!f) return BC_ERROR_INVALID_VALUE;
   fprintf(f, "# log_n log_cols contig_log num_strided strided_bpw contig_grid ragged_unit_contig "
-             "ragged_unit_strided\n");
+             "ragged_unit_strided tile_log\n");
   {
     std::lock_guard<std::mutex> lk(g_tune_mu);
     for (const auto &kv : g_tune) {
       const TuneParams &t = kv.second;
-      fprintf(f, "%u %u %u %u %u %u %u %u\n", kv.first >> 8, kv.first & 0xFF, t.contig_log, t.num_strided,
-              t.strided_bpw, t.contig_grid, t.unit_contig, t.unit_strided);
+      fprintf(f, "%u %u %u %u %u %u %u %u %u\n", kv.first >> 8, kv.first & 0xFF, t.contig_log, t.num_strided,
+              t.strided_bpw, t.contig_grid, t.unit_contig, t.unit_strided, t.tile_log);
     }
   }
   return fclose(f) == 0 ? BC_SUCCESS : BC_ERROR_INVALID_VALUE;

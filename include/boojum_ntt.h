@@ -355,6 +355,9 @@ typedef struct {
   uint32_t contig_grid;         /* grid cap of contiguous passes */
   uint32_t ragged_unit_contig;  /* ragged work unit: tiles, <= 64 */
   uint32_t ragged_unit_strided; /* ragged work unit: (column, block) pairs, <= 64 */
+  uint32_t tile_log;            /* workgroup tile: 8 = one-wave 256-point passes (log_n 16, no coset, up to 65535
+                                   columns), 12 = 4096-point pass kernels; 0: built-in (256-point for 1..4
+                                   columns of 2^16) */
 } bc_tune_params;
 
 /* One HBM pass of a planned transform, as bc_batch_ntt_internal would launch
@@ -368,7 +371,7 @@ typedef struct {
   uint64_t positions;           /* strided: r-ranges (grid.x) */
   uint64_t groups;              /* strided: pair groups (grid.y, <= 65535) */
   uint32_t pairs_per_workgroup; /* strided */
-  uint32_t reserved;
+  uint32_t workgroup_threads;   /* 256, or 64 for the one-wave 256-point passes */
 } bc_pass_info;
 
 /* The pass plan of bc_batch_ntt_internal for this shape (log_n >= 8; below
@@ -385,8 +388,9 @@ int bc_tune_set(uint32_t log_n, uint32_t log_cols, const bc_tune_params *params)
 /* The parameters a call with this shape uses (all zero: built-in). */
 int bc_tune_get(uint32_t log_n, uint32_t num_cols, bc_tune_params *params);
 int bc_tune_clear(void);
-/* Text database, one entry per line: log_n log_cols and the six fields in
- * order; '#' starts a comment line.  Load merges into the table. */
+/* Text database, one entry per line: log_n log_cols and the seven fields in
+ * order (a line of six fields leaves tile_log 0); '#' starts a comment line.
+ * Load merges into the table. */
 int bc_tune_load(const char *path);
 int bc_tune_save(const char *path);
 

@@ -152,3 +152,21 @@ def test_plan_rejects_reference_asserts():
     with pytest.raises(AssertionError):
         ntt.describe_plan(20, 1, log_extension_degree=2, coset_index=4)
     assert ntt.describe_plan(7, 4) == []
+
+
+def test_plan_of_the_latency_path():
+    """1..4 columns of 2^16 without a coset take two one-wave 8-bit passes
+    (the top one strided, with n^-1 for inverses); 5 columns, or a coset,
+    take the 4096-point pass kernels."""
+    from boojum_cuda_amd import ntt
+    for cols in (1, 4):
+        for br, inv in [(False, False), (True, True)]:
+            plan = ntt.describe_plan(16, cols, br, inv)
+            assert [p["log_h"] for p in plan] == [8, 8]
+            assert all(p["workgroup_threads"] == 64 for p in plan)
+            top = plan[1 if br else 0]
+            assert top["contiguous"] == 0 and top["positions"] == 256 and top["groups"] == cols
+            assert top["mode"] == (1 if inv else 0)
+            assert plan[0 if br else 1]["contiguous"] == 1
+    assert all(p["workgroup_threads"] == 256 for p in ntt.describe_plan(16, 5))
+    assert all(p["workgroup_threads"] == 256 for p in ntt.describe_plan(16, 1, log_extension_degree=1))

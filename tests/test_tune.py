@@ -25,12 +25,12 @@ def test_splits_cover_valid_plans():
 
 def test_set_get_nearest_bucket(tune):
     P = tune.TuneParams
-    assert tune.get_entry(20, 256).key() == (0,) * 6
+    assert tune.get_entry(20, 256).key() == (0,) * 7
     tune.set_entry(20, 8, P(10, 2, 4, 1024, 2, 4))
-    assert tune.get_entry(20, 256).key() == (10, 2, 4, 1024, 2, 4)
-    assert tune.get_entry(20, 300).key() == (10, 2, 4, 1024, 2, 4)   # bucket 8
-    assert tune.get_entry(20, 1).key() == (10, 2, 4, 1024, 2, 4)     # nearest bucket of log_n 20
-    assert tune.get_entry(21, 256).key() == (0,) * 6                  # other log_n: built-in
+    assert tune.get_entry(20, 256).key() == (10, 2, 4, 1024, 2, 4, 0)
+    assert tune.get_entry(20, 300).key() == (10, 2, 4, 1024, 2, 4, 0)   # bucket 8
+    assert tune.get_entry(20, 1).key() == (10, 2, 4, 1024, 2, 4, 0)     # nearest bucket of log_n 20
+    assert tune.get_entry(21, 256).key() == (0,) * 7                  # other log_n: built-in
     tune.set_entry(20, 0, P(12, 1, 0, 0, 0, 0))
     assert tune.get_entry(20, 1).key()[:2] == (12, 1)
     assert tune.get_entry(20, 16).key()[:2] == (12, 1)                # bucket 4: 0 and 8 tie, smaller wins
@@ -47,6 +47,8 @@ def test_set_get_nearest_bucket(tune):
     (20, (0, 1, 0, 0, 0, 0)),     # strided count without a split
     (20, (0, 0, 65, 0, 0, 0)),    # pairs per workgroup too large
     (20, (0, 0, 0, 0, 65, 0)),    # ragged unit too large
+    (20, (0, 0, 0, 0, 0, 0, 8)),  # one-wave 256-point passes only for 2^16
+    (16, (0, 0, 0, 0, 0, 0, 10)), # tile must be 8 or 12
 ])
 def test_invalid_entries_rejected(tune, log_n, params):
     with pytest.raises(RuntimeError):
@@ -55,13 +57,14 @@ def test_invalid_entries_rejected(tune, log_n, params):
 
 def test_save_load_round_trip(tune, tmp_path):
     P = tune.TuneParams
-    entries = {(20, 8): (10, 2, 4, 1024, 2, 4), (16, 0): (10, 1, 0, 0, 1, 1), (12, 5): (12, 0, 0, 4096, 4, 0)}
+    entries = {(20, 8): (10, 2, 4, 1024, 2, 4, 0), (16, 0): (10, 1, 0, 0, 1, 1, 0), (12, 5): (12, 0, 0, 4096, 4, 0, 0),
+               (16, 5): (0, 0, 0, 0, 0, 0, 8), (16, 1): (0, 0, 0, 0, 0, 0, 12)}
     for (log_n, lc), v in entries.items():
         tune.set_entry(log_n, lc, P(*v))
     path = str(tmp_path / "db.txt")
     tune.save(path)
     tune.clear()
-    assert tune.get_entry(20, 256).key() == (0,) * 6
+    assert tune.get_entry(20, 256).key() == (0,) * 7
     tune.load(path)
     for (log_n, lc), v in entries.items():
         assert tune.get_entry(log_n, 1 << lc).key() == v
@@ -77,7 +80,7 @@ def test_malformed_database_rejected(tune, tmp_path):
     bad.write_text("20 8 8 2 0 0 0 0\n")          # invalid split
     with pytest.raises(RuntimeError):
         tune.load(str(bad))
-    assert tune.get_entry(20, 256).key() == (0,) * 6
+    assert tune.get_entry(20, 256).key() == (0,) * 7
     with pytest.raises(RuntimeError):
         tune.load(str(tmp_path / "missing.txt"))
 

@@ -8,7 +8,7 @@ import oracle
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("log_n,cols", [(13, 3), (16, 2), (20, 2), (22, 1)])
+@pytest.mark.parametrize("log_n,cols", [(13, 3), (16, 2), (16, 8), (20, 2), (22, 1)])
 def test_tuned_plans_match_builtin(gpu_ctx, tmp_path, log_n, cols):
     import torch
     from boojum_cuda_amd import autotune, ntt
@@ -35,6 +35,8 @@ def test_tuned_plans_match_builtin(gpu_ctx, tmp_path, log_n, cols):
                                   oracle.ntt_column(col, log_n, False, False))
         assert np.array_equal(ref_z.cpu().numpy().view(np.uint64), oracle.canon(host))  # round trip
         variants = [autotune.TuneParams(c, k, 0, 0, 0, 0) for c, k in autotune.splits(log_n)]
+        if log_n == 16:  # workgroup tile: one-wave 256-point passes and 4096-point pass kernels
+            variants += [autotune.TuneParams(0, 0, 0, 0, 0, 0, t) for t in (8, 12)]
         variants += [autotune.TuneParams(0, 0, bpw, grid, 0, 0) for bpw in (1, 3, 32) for grid in (1, 700)]
         for p in variants:
             autotune.set_entry(log_n, lc, p)
