@@ -172,6 +172,26 @@ def bench_serve(count, rate, budget_bytes, seed=5):
                          "frac": gbs / HBM_PEAK_GBS}}
 
 
+def bench_serve_sweep(rates, count, budget_bytes):
+    """The serving leg at several offered loads (the same request
+    distribution): served requests per second and latency per load, the
+    capacity (most requests per second served) and the knee, the highest
+    offered load still served at >= 80% of its rate (with `count` requests per
+    point the last requests' own service time is part of the makespan, so an
+    unsaturated point serves about 90% of its offered rate)."""
+    points = []
+    for rate in rates:
+        r = bench_serve(count, rate, budget_bytes)
+        points.append({"offered_req_per_s": rate, "served_req_per_s": r["requests_per_s"],
+                       "p50_latency_ms": r["p50_latency_ms"], "p99_latency_ms": r["p99_latency_ms"],
+                       "output_elems_per_s": r["output_elems_per_s"], "makespan_s": r["makespan_s"]})
+        progress(f"serve sweep {rate}/s: served {r['requests_per_s']:.1f}/s, p50 {r['p50_latency_ms']:.1f} ms, "
+                 f"p99 {r['p99_latency_ms']:.1f} ms")
+    kept = [p["offered_req_per_s"] for p in points if p["served_req_per_s"] >= 0.8 * p["offered_req_per_s"]]
+    return {"requests_per_point": count, "points": points, "knee_req_per_s": max(kept) if kept else None,
+            "capacity_req_per_s": max(p["served_req_per_s"] for p in points)}
+
+
 def bench_ops(log_n, cols, reps):
     """Order conversion and domain powers around the NTT on the batched
     shape: bit_reverse (out of place and in place, 16 B per element moved),
@@ -438,6 +458,9 @@ def other_legs(args):
         extras["serve"] = bench_serve(args.serve_requests, args.serve_rate, int(args.serve_budget_gb * 1e9))
         progress(f"serving done: p50 {extras['serve']['p50_latency_ms']:.1f} ms, "
                  f"p99 {extras['serve']['p99_latency_ms']:.1f} ms")
+        if not args.no_serve_sweep:
+            torch.cuda.empty_cache()
+            extras["serve"]["sweep"] = bench_serve_sweep((10, 25, 50, 100, 200), 60, int(args.serve_budget_gb * 1e9))
     return extras
 
 
@@ -466,6 +489,7 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=100)
     ap.add_argument("--serve-rate", type=float, default=25.0, help="offered load, requests per second")
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
+    ap.add_argument("--no-serve-sweep", action="store_true", help="skip the serving offered-load sweep")
     ap.add_argument("--legs-only", action="store_true",
                     help="skip the batched-NTT measurement and print only the other legs (for per-leg profiling)")
     ap.add_argument("--mode", choices=["ntt", "copy"], default="ntt",
