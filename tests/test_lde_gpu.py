@@ -37,7 +37,8 @@ def expected_lde(evals, log_n, k, bitrev_inputs):
 
 
 @pytest.mark.parametrize("log_n,k,bitrev", [(3, 1, True), (8, 2, True), (10, 2, False), (13, 3, True),
-                                             (16, 1, False), (17, 2, True)])
+                                             (16, 1, False), (17, 2, True), (14, 1, True), (15, 4, True),
+                                             (21, 2, True)])
 def test_batch_lde_matches_oracle(gpu_ctx, log_n, k, bitrev):
     from boojum_cuda_amd import lde
     n, cols = 1 << log_n, 3
