@@ -8,11 +8,14 @@ namespace bc {
 namespace {
 using namespace pass;
 
-// Three workgroups per CU: the 12-bit pass then fits in 168 VGPRs (a few
-// loop-invariant dwords spill) and runs 3 waves per SIMD instead of 2, which
-// hides more of its LDS exchanges (6% faster on 256 x 2^20).
+// Three workgroups per CU for the DIF (n2b) variants: the 12-bit pass then
+// fits in 168 VGPRs (a few loop-invariant dwords spill) and runs 3 waves per
+// SIMD instead of 2, which hides more of its LDS exchanges (6% faster on
+// 256 x 2^20).  The DIT (b2n) variants, which stage their loads through one
+// more LDS exchange, spill 15-20 VGPRs at that bound and run 1-3% faster at
+// two workgroups per CU without spills.
 template <int LH, bool INV, bool DIT, int MODE, bool MC>
-__global__ __launch_bounds__(NT, 3) void ntt_contig_kernel(FastArgs a) {
+__global__ __launch_bounds__(NT, DIT ? 2 : 3) void ntt_contig_kernel(FastArgs a) {
   __shared__ PassLds sh;
   contig_run<LH, INV, DIT, MODE, MC>(a, sh, blockIdx.x, a.tiles, gridDim.x);
 }
