@@ -8,7 +8,10 @@
 //
 //   contig<h>  (8 <= h <= 12): R = 1, the last pass in n2b order.  A tile is
 //              4096 consecutive points = 2^(12-h) sub-DFTs.  Levels of radix
-//              16, 16 and 2^(h-8) run in registers, two LDS exchanges.
+//              16, 16 and 2^(h-8) run in registers, with LDS exchanges
+//              between them; for plain and n^-1 passes the one between
+//              levels 1 and 2 (and in b2n the load regrouping) stays inside
+//              a wave, with no workgroup barrier.
 //   strided<h> (4 <= h <= 8):  R >= 16, every other pass.  A tile is
 //              2^(12-h) consecutive r times the h-point sub-DFT, so each row of
 //              the tile is >= 128 contiguous bytes.  Levels of radix 16 and
@@ -130,14 +133,36 @@ template <int LH, bool MC, bool CT> struct ContigTile {
   }
 };
 
+// Wave-region mappings of a 4096-point tile, used from level 1 on when a
+// transform has three register levels.  Tile bits 10-11 are the wave index in
+// the level-1 mapping (celem<LH, 1, 4> for every 9 <= LH <= 12), so each wave
+// then owns the 1024 points (and LDS slots) w * 1024 + [0, 1024):
+//   wl_run: thread t holds 16 consecutive points t * 16 + k (the last level's
+//           2^(LH-8)-point runs, grouped by k's low bits);
+//   wl_row: register i of lane l of wave w holds point w * 1024 + i * 64 + l
+//           (coalesced: 64 consecutive points per register).
+__device__ __forceinline__ uint32_t wl_run(uint32_t t, int k) { return (t << 4) + (uint32_t)k; }
+__device__ __forceinline__ uint32_t wl_row(uint32_t t, int i) {
+  return ((t >> 6) << 10) + ((uint32_t)i << 6) + (t & 63u);
+}
+
+// Orders one wave's LDS accesses across its lanes.  A wave's LDS operations
+// execute in program order, so this only keeps the compiler from moving LDS
+// loads and stores across it; no workgroup barrier is involved.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Loads a tile's 16 values per thread in the level-0 mapping (lanes on
-// consecutive points).
-template <int LH, bool MC, bool CT>
+// consecutive points), or (ROW) in the wl_row mapping.
+template <int LH, bool MC, bool CT, bool ROW = false>
 __device__ __forceinline__ void contig_load(const FastArgs &a, uint32_t tile, uint32_t t, uint64_t *x) {
   const ContigTile<LH, MC, CT> ct(a, tile);
 #pragma unroll
   for (int i = 0; i < 16; ++i) {
-    const uint64_t *q = ct.src(a, celem<LH, 0, 4>(t, 0, i));
+    const uint64_t *q = ct.src(a, ROW ? wl_row(t, i) : celem<LH, 0, 4>(t, 0, i));
     x[i] = q ? *q : 0ull;
   }
 }
@@ -159,6 +184,13 @@ __device__ __forceinline__ void contig_run(const FastArgs &a, PassLds &sh, uint3
   constexpr int A2 = NL == 3 ? LH - 8 : 4;  // width of the third level
   constexpr int G2 = 16 >> A2;              // third-level groups per thread
   constexpr int LL = NL - 1, AL = NL == 3 ? A2 : 4, GL = 16 >> AL;  // last level
+  // WL: with three levels and no coset factors, the exchange between levels 1
+  // and 2 stays inside each wave's own 1024 points (wl_run) and needs no
+  // workgroup barrier.  b2n also loads in the wl_row mapping, so its
+  // regrouping exchange is wave-local too (two barriers per tile instead of
+  // four); n2b keeps its store in the level-0 mapping (three barriers), which
+  // measured faster than wl_row stores.
+  constexpr bool WL = NL == 3 && !MC && (MODE == MODE_NONE || MODE == MODE_NINV);
   uint64_t *lds = sh.tile;
   uint64_t *tw1 = sh.tw;
   const uint32_t t = threadIdx.x;
@@ -181,9 +213,11 @@ __device__ __forceinline__ void contig_run(const FastArgs &a, PassLds &sh, uint3
     }
   }
 
+  // b2n with WL loads in the wl_row mapping, everything else in level 0's
+  constexpr bool ROW_LOAD = WL && DIT;
   uint32_t tile = tile_begin;
   uint64_t pf[16];
-  if (tile < tile_end) contig_load<LH, MC, CT>(a, tile, t, pf);
+  if (tile < tile_end) contig_load<LH, MC, CT, ROW_LOAD>(a, tile, t, pf);
   __syncthreads();  // tw1 and coset part factors staged
 
   for (; tile < tile_end; tile += step) {
@@ -193,7 +227,7 @@ __device__ __forceinline__ void contig_run(const FastArgs &a, PassLds &sh, uint3
     for (int i = 0; i < 16; ++i) x[i] = pf[i];
     {
       const uint32_t next = tile + step;
-      if (next < tile_end) contig_load<LH, MC, CT>(a, next, t, pf);  // prefetch: in flight during this tile's math
+      if (next < tile_end) contig_load<LH, MC, CT, ROW_LOAD>(a, next, t, pf);  // prefetch: in flight during this tile's math
     }
     uint64_t cfb = 0;  // coset factor of the thread's point 0 in this tile
     if constexpr (SC) {
@@ -220,7 +254,27 @@ __device__ __forceinline__ void contig_run(const FastArgs &a, PassLds &sh, uint3
 #pragma unroll
       for (int i = 0; i < 16; ++i) x[i] = lds[pidx(celem<LH, 1, 4>(t, 0, i))];
       dft_dif<4, INV>(x);
-      if constexpr (NL == 3) {
+      if constexpr (WL) {
+        level_twist_lds<LH - 4>(x, tw1, rp1);
+        // each thread rewrites the slots it read, inside its wave's region
+#pragma unroll
+        for (int i = 0; i < 16; ++i) lds[pidx(celem<LH, 1, 4>(t, 0, i))] = x[i];
+        wave_lds_sync();
+#pragma unroll
+        for (int k = 0; k < 16; ++k) x[k] = lds[pidx(wl_run(t, k))];
+#pragma unroll
+        for (int g = 0; g < G2; ++g) dft_dif<A2, INV>(x + g * (1 << A2));
+#pragma unroll
+        for (int k = 0; k < 16; ++k) lds[pidx(wl_run(t, k))] = x[k];
+        __syncthreads();  // the level-0 mapping crosses waves
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+          const uint32_t e = celem<LH, 0, 4>(t, 0, i);
+          uint64_t v = lds[pidx(e)];
+          if constexpr (MODE == MODE_NINV) v = gl::mul(v, a.n_inv);
+          *ct.dst(a, e) = v;
+        }
+      } else if constexpr (NL == 3) {
         level_twist_lds<LH - 4>(x, tw1, rp1);
 #pragma unroll
         for (int i = 0; i < 16; ++i) lds[pidx(celem<LH, 1, 4>(t, 0, i))] = x[i];
@@ -235,19 +289,53 @@ __device__ __forceinline__ void contig_run(const FastArgs &a, PassLds &sh, uint3
       // regroup the last level's runs into the coalesced level-0 mapping
       // through LDS (each thread rewrites the slots it read, so no barrier
       // is needed before the write), then store
+      if constexpr (!WL) {
 #pragma unroll
-      for (int g = 0; g < GL; ++g)
+        for (int g = 0; g < GL; ++g)
 #pragma unroll
-        for (int i = 0; i < (1 << AL); ++i) lds[pidx(celem<LH, LL, AL>(t, g, i))] = x[g * (1 << AL) + i];
+          for (int i = 0; i < (1 << AL); ++i) lds[pidx(celem<LH, LL, AL>(t, g, i))] = x[g * (1 << AL) + i];
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+          const uint32_t e = celem<LH, 0, 4>(t, 0, i);
+          uint64_t v = lds[pidx(e)];
+          if constexpr (MODE == MODE_NINV) v = gl::mul(v, a.n_inv);
+          if constexpr (MODE == MODE_LDE_OUT) v = gl::mul(v, gl::mul(cfb, sh.cf[i]));
+          uint64_t *q = ct.dst(a, e);
+          if (q) *q = v;  // canonical: the field ops keep values in [0, p)
+        }
+      }
+    } else if constexpr (WL) {
+      // loaded in the wl_row mapping: regroup into runs inside the wave's region
+#pragma unroll
+      for (int i = 0; i < 16; ++i) lds[pidx(wl_row(t, i))] = x[i];
+      wave_lds_sync();
+#pragma unroll
+      for (int k = 0; k < 16; ++k) x[k] = lds[pidx(wl_run(t, k))];
+#pragma unroll
+      for (int g = 0; g < G2; ++g) dft_dit<A2, INV>(x + g * (1 << A2));
+#pragma unroll
+      for (int k = 0; k < 16; ++k) lds[pidx(wl_run(t, k))] = x[k];
+      wave_lds_sync();
+#pragma unroll
+      for (int i = 0; i < 16; ++i) x[i] = lds[pidx(celem<LH, 1, 4>(t, 0, i))];
+      level_twist_lds<LH - 4>(x, tw1, rp1);
+      dft_dit<4, INV>(x);
+      // each thread rewrites the slots it read; the level-0 mapping crosses waves
+#pragma unroll
+      for (int i = 0; i < 16; ++i) lds[pidx(celem<LH, 1, 4>(t, 0, i))] = x[i];
       __syncthreads();
+#pragma unroll
+      for (int i = 0; i < 16; ++i) x[i] = lds[pidx(celem<LH, 0, 4>(t, 0, i))];
+#pragma unroll
+      for (int i = 1; i < 16; ++i) x[i] = gl::mul(x[i], tw0[(i - 1) << (LH - 4)]);
+      dft_dit<4, INV>(x);
 #pragma unroll
       for (int i = 0; i < 16; ++i) {
         const uint32_t e = celem<LH, 0, 4>(t, 0, i);
-        uint64_t v = lds[pidx(e)];
+        uint64_t v = x[i];
         if constexpr (MODE == MODE_NINV) v = gl::mul(v, a.n_inv);
-        if constexpr (MODE == MODE_LDE_OUT) v = gl::mul(v, gl::mul(cfb, sh.cf[i]));
-        uint64_t *q = ct.dst(a, e);
-        if (q) *q = v;  // canonical: the field ops keep values in [0, p)
+        *ct.dst(a, e) = v;
       }
     } else {
       // regroup the coalesced load into the last level's runs through LDS
