@@ -327,9 +327,14 @@ def bench_poseidon(log_n, cols, reps):
 
 def bench_latency(reps=200):
     """Config 1: one column of 2^16, forward natural -> bit-reversed, out of
-    place.  Reports the GPU time per call (HIP events, back to back), the
-    host-observed latency of a call plus a stream synchronize (median), and
-    whether the inverse round trip returns the input."""
+    place.  Reports
+    - gpu_us_per_call: GPU time per call back to back, from HIP events around
+      the replay of a graph holding `reps` calls (so the host's enqueue cost
+      is not in it); every replay's output is checked against the eager one;
+    - eager_enqueue_us_per_call: the same calls enqueued one by one from
+      Python (bounded by the host's per-call cost when that exceeds the GPU's);
+    - the host-observed latency of a call plus a stream synchronize (median,
+      p99), and whether the inverse round trip returns the input."""
     import torch
     from boojum_cuda_amd import ntt
     log_n = 16
@@ -343,13 +348,38 @@ def bench_latency(reps=200):
     for _ in range(10):
         ntt.batch_ntt_out_of_place(x, y, log_n, 1, 0, 0, n, n, False, False, 0, 0)
     torch.cuda.synchronize()
+    want = y.clone()
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
     ev[0].record(stream)
     for _ in range(reps):
         ntt.batch_ntt_out_of_place(x, y, log_n, 1, 0, 0, n, n, False, False, 0, 0)
     ev[1].record(stream)
     torch.cuda.synchronize()
-    gpu_us = ev[0].elapsed_time(ev[1]) * 1e3 / reps
+    eager_us = ev[0].elapsed_time(ev[1]) * 1e3 / reps
+
+    graph = torch.cuda.CUDAGraph()
+    side = torch.cuda.Stream()
+    side.wait_stream(stream)
+    with torch.cuda.stream(side):
+        with torch.cuda.graph(graph, stream=side):
+            for _ in range(reps):
+                ntt.batch_ntt_out_of_place(x, y, log_n, 1, 0, 0, n, n, False, False, 0, 0)
+    stream.wait_stream(side)
+    torch.cuda.synchronize()
+    replays_exact = True
+    times = []
+    for _ in range(5):
+        y.zero_()
+        torch.cuda.synchronize()
+        ev[0].record(stream)
+        graph.replay()
+        ev[1].record(stream)
+        torch.cuda.synchronize()
+        times.append(ev[0].elapsed_time(ev[1]) * 1e3 / reps)
+        replays_exact = replays_exact and bool(torch.equal(y, want))
+    times.sort()
+    del graph
+
     lat = []
     for _ in range(reps):
         t0 = time.perf_counter()
@@ -359,7 +389,11 @@ def bench_latency(reps=200):
     lat.sort()
     ntt.batch_ntt_out_of_place(y, z, log_n, 1, 0, 0, n, n, True, True, 0, 0)  # b2n inverse
     torch.cuda.synchronize()
-    return {"workload": "1 x 2^16 forward n2b, out of place", "gpu_us_per_call": gpu_us,
+    return {"workload": "1 x 2^16 forward n2b, out of place",
+            "gpu_us_per_call": times[len(times) // 2],
+            "gpu_time_method": f"HIP events around a graph of {reps} back-to-back calls, median of 5 replays",
+            "graph_replays_equal_eager": replays_exact,
+            "eager_enqueue_us_per_call": eager_us,
             "host_latency_us_median": lat[len(lat) // 2], "host_latency_us_p99": lat[int(0.99 * (len(lat) - 1))],
             "inverse_round_trip_exact": bool(torch.equal(z, x))}
 

@@ -130,7 +130,10 @@ template <bool INV> __device__ __forceinline__ void wave_dit256(uint64_t *x, con
 template <bool INV, bool DIT>
 __global__ __launch_bounds__(SW) void ntt_small_top_kernel(FastArgs a) {
   __shared__ WaveLds sh;
-  const uint32_t l = threadIdx.x, r = blockIdx.x;
+  const uint32_t l = threadIdx.x;
+  // workgroups are dispatched round-robin over the 8 XCDs: keep the 16
+  // workgroups whose r share a 128-byte line on one XCD (one L2)
+  const uint32_t r = ((blockIdx.x & 7u) << 5) | (blockIdx.x >> 3);
   const uint64_t c = blockIdx.y;
   const uint64_t *in = a.in + c * a.stride_in + r;
   uint64_t *out = a.out + c * a.stride_out + r;

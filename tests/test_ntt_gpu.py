@@ -208,3 +208,40 @@ def test_latency_path_equals_batched_path(gpu_ctx):
             ntt.batch_ntt_out_of_place(x, y4, 16, 4, 0, 0, n, n, bitrev, inverse, 0, 0)
             ntt.batch_ntt_out_of_place(x, y5, 16, 5, 0, 0, n, n, bitrev, inverse, 0, 0)
             assert np.array_equal(to_host(y4), to_host(y5)[:4 * n]), (bitrev, inverse)
+
+
+def test_latency_path_graph_replays_match_eager(gpu_ctx):
+    """The latency path captured in a graph (forward n2b x -> y, then b2n
+    inverse y -> z) gives the eager results on every replay, with eager work
+    of other sizes between the replays."""
+    import torch
+    from boojum_cuda_amd import ntt
+    n = 1 << 16
+    host = oracle.fill_raw(n, 31, 0)
+    x = to_dev(host)
+    y, z = torch.empty_like(x), torch.empty_like(x)
+    ntt.batch_ntt_out_of_place(x, y, 16, 1, 0, 0, n, n, False, False, 0, 0)
+    ntt.batch_ntt_out_of_place(y, z, 16, 1, 0, 0, n, n, True, True, 0, 0)
+    want_y, want_z = to_host(y).copy(), to_host(z).copy()
+    assert np.array_equal(want_y, oracle.ntt_column(host, 16, False, False))
+    assert np.array_equal(want_z, oracle.canon(host))
+    graph = torch.cuda.CUDAGraph()
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        with torch.cuda.graph(graph, stream=side):
+            ntt.batch_ntt_out_of_place(x, y, 16, 1, 0, 0, n, n, False, False, 0, 0)
+            ntt.batch_ntt_out_of_place(y, z, 16, 1, 0, 0, n, n, True, True, 0, 0)
+    torch.cuda.current_stream().wait_stream(side)
+    torch.cuda.synchronize()
+    other = to_dev(oracle.fill_raw(4 << 12, 5, 0))
+    other_out = torch.empty_like(other)
+    for k in range(4):
+        y.zero_()
+        z.zero_()
+        graph.replay()
+        torch.cuda.synchronize()
+        assert np.array_equal(to_host(y), want_y), k
+        assert np.array_equal(to_host(z), want_z), k
+        ntt.batch_ntt_out_of_place(other, other_out, 12, 4, 0, 0, 1 << 12, 1 << 12, False, k % 2 == 1, 0, 0)
+        torch.cuda.synchronize()
