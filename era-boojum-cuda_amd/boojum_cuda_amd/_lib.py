@@ -60,6 +60,10 @@ SIGNATURES = {
     "bc_poseidon2_nodes": (c_int, [c_vp, c_u64, c_vp, c_u64, c_vp]),
     "bc_poseidon2_build_merkle_tree_nodes": (c_int, [c_vp, c_u64, c_vp, c_u64, c_u32, c_vp]),
     "bc_poseidon2_build_merkle_tree": (c_int, [c_vp, c_u64, c_vp, c_u64, c_u32, c_u32, c_vp]),
+    "bc_poseidon_leaves_cooperative": (c_int, [c_vp, c_u64, c_vp, c_u64, c_u32, c_bool, c_bool, c_vp]),
+    "bc_poseidon2_leaves_cooperative": (c_int, [c_vp, c_u64, c_vp, c_u64, c_u32, c_bool, c_bool, c_vp]),
+    "bc_poseidon_nodes_cooperative": (c_int, [c_vp, c_u64, c_vp, c_u64, c_vp]),
+    "bc_poseidon2_nodes_cooperative": (c_int, [c_vp, c_u64, c_vp, c_u64, c_vp]),
     "bc_gather_rows": (c_int, [c_vp, c_u32, c_u32, c_vp, c_u64, c_u64, c_vp, c_u64, c_u64, c_u32, c_vp]),
     "bc_gather_merkle_paths": (c_int, [c_vp, c_u32, c_vp, c_u64, c_vp, c_u64, c_u32, c_vp]),
     "bc_fill_seeded": (c_int, [c_vp, c_u32, c_u32, c_u64, c_u64, c_u32, c_vp]),

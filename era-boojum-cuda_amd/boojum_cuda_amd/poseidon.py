@@ -89,6 +89,21 @@ def launch_single_thread_nodes_kernel(values, results, stream=None, p=POSEIDON):
                            _stream(stream)), "launch_nodes_kernel")
 
 
+def launch_cooperative_leaves_kernel(values, results, log_rows_per_hash, load_intermediate=False,
+                                     store_intermediate=False, stream=None, p=POSEIDON2):
+    """src/poseidon.rs:194-216: the same leaves with 16 lanes per permutation
+    (the reference has it for Poseidon2 only; both hashes here)."""
+    _check(_fn(p, "leaves_cooperative")(values.data_ptr(), values.numel(), results.data_ptr(), results.numel(),
+                                        log_rows_per_hash, bool(load_intermediate), bool(store_intermediate),
+                                        _stream(stream)), "launch_leaves_kernel")
+
+
+def launch_cooperative_nodes_kernel(values, results, stream=None, p=POSEIDON2):
+    """src/poseidon.rs:240-260."""
+    _check(_fn(p, "nodes_cooperative")(values.data_ptr(), values.numel(), results.data_ptr(), results.numel(),
+                                       _stream(stream)), "launch_nodes_kernel")
+
+
 def build_merkle_tree_leaves(values, results, log_rows_per_hash, load_intermediate=False,
                              store_intermediate=False, stream=None, p=POSEIDON):
     """src/poseidon.rs:288-309."""

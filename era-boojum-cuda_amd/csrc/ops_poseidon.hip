@@ -409,6 +409,46 @@ __global__ __launch_bounds__(256) void poseidon_nodes_coop_kernel(const uint64_t
   if (j < CAP) results[g + (uint64_t)j * count] = x;
 }
 
+// Cooperative leaves (src/poseidon.rs:194-216): the absorption of
+// poseidon_leaves_kernel with one state over a 16-lane group; for each chunk,
+// lane j < 8 loads position off + j (= c rows + r) into state element j.
+template <int H>
+__global__ __launch_bounds__(256) void poseidon_leaves_coop_kernel(const uint64_t *__restrict__ values,
+                                                                   uint64_t *__restrict__ results,
+                                                                   uint32_t log_rows, uint32_t cols, uint32_t count,
+                                                                   bool load_intermediate, bool store_intermediate) {
+  __shared__ PosTable lt;
+  {
+    const uint64_t *src = (const uint64_t *)&pos_table;
+    uint64_t *dst = (uint64_t *)&lt;
+    for (uint32_t i = threadIdx.x; i < sizeof(PosTable) / sizeof(uint64_t); i += blockDim.x) dst[i] = src[i];
+  }
+  __syncthreads();
+  const uint32_t g = blockIdx.x * (blockDim.x / COOP_LANES) + threadIdx.x / COOP_LANES;
+  const uint32_t j = threadIdx.x % COOP_LANES;
+  if (g >= count) return;  // whole groups leave together
+  const uint64_t rows = 1ull << log_rows;
+  uint64_t x = 0;
+  if (load_intermediate && j >= RATE && j < PT) x = gl::canon(results[g + (uint64_t)(j - RATE) * count]);
+  const uint64_t col_stride = rows * count;
+  const uint64_t *base = values + (uint64_t)g * rows;
+  const uint64_t total = rows * cols;
+  for (uint64_t off = 0; off < total; off += RATE) {
+    if (j < RATE) {
+      const uint64_t k = off + j;
+      x = k < total ? gl::canon(base[(k & (rows - 1)) + (k >> log_rows) * col_stride]) : 0ull;
+    }
+    if constexpr (H == 0) coop_permute(x, j, lt);
+    else coop_permute2(x, j, lt);
+  }
+  // the digest is state[0..4], the stored capacity state[8..12]
+  if (store_intermediate) {
+    if (j >= RATE && j < PT) results[g + (uint64_t)(j - RATE) * count] = x;
+  } else if (j < CAP) {
+    results[g + (uint64_t)j * count] = x;
+  }
+}
+
 // ---- Merkle openings: row and path gathers (native/poseidon_common.cu:9-37) --
 
 // result[col][i rpi + r] = values[col][indexes[i] rpi + r] for r < rpi =
@@ -561,18 +601,28 @@ int require_table() {
 
 using TableUse = std::shared_lock<std::shared_mutex>;
 
-template <int H>
-int launch_leaves(const uint64_t *values, uint64_t *results, uint32_t rows, uint32_t cols, uint32_t count, bool load,
-                  bool store, hipStream_t s) {
+template <int H, bool COOP>
+int launch_leaves(const uint64_t *values, uint64_t *results, uint32_t log_rows, uint32_t cols, uint32_t count,
+                  bool load, bool store, hipStream_t s) {
   if (count == 0) return BC_SUCCESS;
-  poseidon_leaves_kernel<H><<<dim3((count + 255) / 256), dim3(256), 0, s>>>(values, results, rows, cols, count, load,
-                                                                           store);
+  if constexpr (COOP) {
+    constexpr uint32_t per_block = 256 / COOP_LANES;
+    poseidon_leaves_coop_kernel<H><<<dim3((count + per_block - 1) / per_block), dim3(256), 0, s>>>(
+        values, results, log_rows, cols, count, load, store);
+  } else {
+    poseidon_leaves_kernel<H><<<dim3((count + 255) / 256), dim3(256), 0, s>>>(values, results, 1u << log_rows, cols,
+                                                                             count, load, store);
+  }
   return hipGetLastError();
 }
 
-template <int H> int launch_nodes(const uint64_t *values, uint64_t *results, uint32_t count, hipStream_t s) {
+// COOP: always the cooperative kernel (the reference's
+// launch_cooperative_nodes_kernel); otherwise the cooperative kernel for
+// layers of at most COOP_MAX_NODES nodes and one thread per node above.
+template <int H, bool COOP = false>
+int launch_nodes(const uint64_t *values, uint64_t *results, uint32_t count, hipStream_t s) {
   if (count == 0) return BC_SUCCESS;
-  if (count <= COOP_MAX_NODES) {
+  if (COOP || count <= COOP_MAX_NODES) {
     constexpr uint32_t per_block = 256 / COOP_LANES;
     poseidon_nodes_coop_kernel<H><<<dim3((count + per_block - 1) / per_block), dim3(256), 0, s>>>(values, results,
                                                                                                  count);
@@ -585,7 +635,7 @@ template <int H> int launch_nodes(const uint64_t *values, uint64_t *results, uin
 // The entry points of both hashes share these (src/poseidon.rs is generic
 // over the hash in the same way).
 
-template <int H>
+template <int H, bool COOP = false>
 int leaves_impl(const uint64_t *values, uint64_t values_len, uint64_t *results, uint64_t results_len,
                 uint32_t log_rows_per_hash, bool load_intermediate, bool store_intermediate, void *stream) {
   // src/poseidon.rs:139-155
@@ -600,11 +650,11 @@ int leaves_impl(const uint64_t *values, uint64_t values_len, uint64_t *results, 
   if (store_intermediate && (rows * cols) % RATE != 0) return BC_ERROR_INVALID_VALUE;
   const int rc = require_table();
   if (rc != BC_SUCCESS) return rc;
-  return launch_leaves<H>(values, results, (uint32_t)rows, (uint32_t)cols, (uint32_t)count, load_intermediate,
-                          store_intermediate, (hipStream_t)stream);
+  return launch_leaves<H, COOP>(values, results, log_rows_per_hash, (uint32_t)cols, (uint32_t)count,
+                                load_intermediate, store_intermediate, (hipStream_t)stream);
 }
 
-template <int H>
+template <int H, bool COOP = false>
 int nodes_impl(const uint64_t *values, uint64_t values_len, uint64_t *results, uint64_t results_len, void *stream) {
   // src/poseidon.rs:224-232
   if (values_len % RATE != 0 || results_len % CAP != 0 || values_len != 2 * results_len)
@@ -614,7 +664,7 @@ int nodes_impl(const uint64_t *values, uint64_t values_len, uint64_t *results, u
   if (!values || !results || count > 0xFFFFFFFFull) return BC_ERROR_INVALID_VALUE;
   const int rc = require_table();
   if (rc != BC_SUCCESS) return rc;
-  return launch_nodes<H>(values, results, (uint32_t)count, (hipStream_t)stream);
+  return launch_nodes<H, COOP>(values, results, (uint32_t)count, (hipStream_t)stream);
 }
 
 template <int H>
@@ -750,6 +800,34 @@ extern "C" int bc_poseidon2_build_merkle_tree(const uint64_t *values, uint64_t v
                                               uint32_t layers_count, void *stream) {
   const bc::TableUse use(bc::pos_mu);
   return bc::tree_impl<1>(values, values_len, results, results_len, log_rows_per_hash, layers_count, stream);
+}
+
+extern "C" int bc_poseidon_leaves_cooperative(const uint64_t *values, uint64_t values_len, uint64_t *results,
+                                              uint64_t results_len, uint32_t log_rows_per_hash,
+                                              bool load_intermediate, bool store_intermediate, void *stream) {
+  const bc::TableUse use(bc::pos_mu);
+  return bc::leaves_impl<0, true>(values, values_len, results, results_len, log_rows_per_hash, load_intermediate,
+                                  store_intermediate, stream);
+}
+
+extern "C" int bc_poseidon2_leaves_cooperative(const uint64_t *values, uint64_t values_len, uint64_t *results,
+                                               uint64_t results_len, uint32_t log_rows_per_hash,
+                                               bool load_intermediate, bool store_intermediate, void *stream) {
+  const bc::TableUse use(bc::pos_mu);
+  return bc::leaves_impl<1, true>(values, values_len, results, results_len, log_rows_per_hash, load_intermediate,
+                                  store_intermediate, stream);
+}
+
+extern "C" int bc_poseidon_nodes_cooperative(const uint64_t *values, uint64_t values_len, uint64_t *results,
+                                             uint64_t results_len, void *stream) {
+  const bc::TableUse use(bc::pos_mu);
+  return bc::nodes_impl<0, true>(values, values_len, results, results_len, stream);
+}
+
+extern "C" int bc_poseidon2_nodes_cooperative(const uint64_t *values, uint64_t values_len, uint64_t *results,
+                                              uint64_t results_len, void *stream) {
+  const bc::TableUse use(bc::pos_mu);
+  return bc::nodes_impl<1, true>(values, values_len, results, results_len, stream);
 }
 
 extern "C" int bc_gather_rows(const uint32_t *indexes, uint32_t indexes_count, uint32_t log_rows_per_index,

@@ -311,6 +311,27 @@ int bc_poseidon2_build_merkle_tree(const uint64_t *values, uint64_t values_len, 
                                    uint64_t results_len, uint32_t log_rows_per_hash, uint32_t layers_count,
                                    void *stream);
 
+/* launch_cooperative_leaves_kernel (src/poseidon.rs:194-216): the leaves of
+ * bc_poseidon_leaves / bc_poseidon2_leaves (same layout, argument checks and
+ * results) computed with one permutation state spread over 16 lanes, which
+ * finishes sooner when there are few leaves.  The reference implements it for
+ * Poseidon2 only (src/poseidon.rs:91-93); here both hashes have it. */
+int bc_poseidon_leaves_cooperative(const uint64_t *values, uint64_t values_len, uint64_t *results,
+                                   uint64_t results_len, uint32_t log_rows_per_hash, bool load_intermediate,
+                                   bool store_intermediate, void *stream);
+int bc_poseidon2_leaves_cooperative(const uint64_t *values, uint64_t values_len, uint64_t *results,
+                                    uint64_t results_len, uint32_t log_rows_per_hash, bool load_intermediate,
+                                    bool store_intermediate, void *stream);
+
+/* launch_cooperative_nodes_kernel (src/poseidon.rs:240-260): the nodes of
+ * bc_poseidon_nodes / bc_poseidon2_nodes, always with 16 lanes per
+ * permutation (bc_poseidon_nodes itself switches to them for layers of at
+ * most 2^14 nodes, as build_merkle_tree_nodes does, src/poseidon.rs:279-283). */
+int bc_poseidon_nodes_cooperative(const uint64_t *values, uint64_t values_len, uint64_t *results,
+                                  uint64_t results_len, void *stream);
+int bc_poseidon2_nodes_cooperative(const uint64_t *values, uint64_t values_len, uint64_t *results,
+                                   uint64_t results_len, void *stream);
+
 /* gather_rows (src/poseidon.rs:330-358): for each of indexes_count indexes,
  * its 2^log_rows_per_index rows (index * 2^log_rows_per_index + r) of every
  * column of the column-major matrix values (column c at values + c *

@@ -113,6 +113,86 @@ def test_nodes_both_kernels(consts, p, count):
 
 
 @pytest.mark.parametrize("p", HASHES)
+@pytest.mark.parametrize("log_rows", [0, 1, 2, 3])
+@pytest.mark.parametrize("cols", [1, 3, 8, 13])
+def test_cooperative_leaves_match_oracle(consts, p, log_rows, cols):
+    """src/poseidon.rs:574-577 (poseidon2_cooperative_leaves, the test_leaves
+    shapes) and 696-749 (cooperative equals single-thread): the cooperative
+    kernel against the oracle and the single-thread kernel.  count 301 leaves
+    a part-filled last workgroup."""
+    import torch
+    from boojum_cuda_amd import poseidon
+    rows, count = 1 << log_rows, 301
+    vals = np.random.default_rng(log_rows * 37 + cols).integers(0, 2**64, rows * cols * count, dtype=np.uint64)
+    dv = to_dev(vals)
+    coop = torch.zeros(4 * count, dtype=torch.int64, device="cuda")
+    single = torch.zeros(4 * count, dtype=torch.int64, device="cuda")
+    poseidon.launch_cooperative_leaves_kernel(dv, coop, log_rows, p=p)
+    poseidon.launch_single_thread_leaves_kernel(dv, single, log_rows, p=p)
+    got = to_host(coop)
+    assert np.array_equal(got, oracle_leaves(vals, rows, cols, count, consts, p))
+    assert np.array_equal(got, to_host(single))
+
+
+@pytest.mark.parametrize("p", HASHES)
+@pytest.mark.parametrize("rows,c1,c2", [(1, 8, 5), (2, 4, 3), (8, 1, 2)])
+def test_cooperative_leaves_checkpointed(consts, p, rows, c1, c2):
+    """src/poseidon.rs:578-581 (poseidon2_cooperative_leaves_checkpointed):
+    capacity stored by one call and loaded by the next, mixing the cooperative
+    and single-thread kernels across the checkpoint."""
+    import torch
+    from boojum_cuda_amd import poseidon
+    count = 130
+    log_rows = rows.bit_length() - 1
+    vals = np.random.default_rng(rows * 11 + c1).integers(0, P, rows * (c1 + c2) * count, dtype=np.uint64)
+    split = rows * c1 * count
+    want_mid = oracle_leaves(vals[:split], rows, c1, count, consts, p, store_intermediate=True)
+    want = oracle_leaves(vals, rows, c1 + c2, count, consts, p)
+    for first, second in ((poseidon.launch_cooperative_leaves_kernel, poseidon.launch_cooperative_leaves_kernel),
+                          (poseidon.launch_cooperative_leaves_kernel, poseidon.launch_single_thread_leaves_kernel),
+                          (poseidon.launch_single_thread_leaves_kernel, poseidon.launch_cooperative_leaves_kernel)):
+        res = torch.zeros(4 * count, dtype=torch.int64, device="cuda")
+        first(to_dev(vals[:split]), res, log_rows, store_intermediate=True, p=p)
+        assert np.array_equal(to_host(res), want_mid)
+        second(to_dev(vals[split:]), res, log_rows, load_intermediate=True, p=p)
+        assert np.array_equal(to_host(res), want)
+
+
+@pytest.mark.parametrize("p", HASHES)
+@pytest.mark.parametrize("count", [1, 2, 255, 1000, 20000])
+def test_cooperative_nodes_match_oracle(consts, p, count):
+    """src/poseidon.rs:624-626 (poseidon2_cooperative_nodes) and 761-804
+    (cooperative equals single-thread), including a layer above 2^14 nodes,
+    where the single-thread entry point runs one lane per node."""
+    import torch
+    from boojum_cuda_amd import poseidon
+    vals = np.random.default_rng(count + 7).integers(0, 2**64, 8 * count, dtype=np.uint64)
+    dv = to_dev(vals)
+    coop = torch.zeros(4 * count, dtype=torch.int64, device="cuda")
+    single = torch.zeros(4 * count, dtype=torch.int64, device="cuda")
+    poseidon.launch_cooperative_nodes_kernel(dv, coop, p=p)
+    poseidon.launch_single_thread_nodes_kernel(dv, single, p=p)
+    got = to_host(coop)
+    assert np.array_equal(got, oracle_nodes(vals, count, consts, p))
+    assert np.array_equal(got, to_host(single))
+
+
+def test_cooperative_bad_arguments_raise(consts):
+    """The cooperative entry points check their arguments as the single-thread
+    ones do (src/poseidon.rs:139-155, 248-255)."""
+    import torch
+    from boojum_cuda_amd import poseidon
+    vals = torch.zeros(8 * 3, dtype=torch.int64, device="cuda")
+    with pytest.raises(AssertionError):  # results not whole digests
+        poseidon.launch_cooperative_leaves_kernel(vals, torch.zeros(5, dtype=torch.int64, device="cuda"), 0)
+    with pytest.raises(AssertionError):  # 6 values per leaf cannot store a capacity checkpoint
+        poseidon.launch_cooperative_leaves_kernel(vals, torch.zeros(4 * 4, dtype=torch.int64, device="cuda"), 0,
+                                                  store_intermediate=True, p="poseidon")
+    with pytest.raises(AssertionError):  # values must be twice the results
+        poseidon.launch_cooperative_nodes_kernel(vals, torch.zeros(4 * 5, dtype=torch.int64, device="cuda"))
+
+
+@pytest.mark.parametrize("p", HASHES)
 @pytest.mark.parametrize("log_leaves,log_rows,cols", [(0, 0, 3), (4, 0, 8), (10, 1, 5), (12, 0, 16)])
 def test_merkle_tree_matches_oracle(consts, p, log_leaves, log_rows, cols):
     """src/poseidon.rs:628-694 (merkle_tree): every layer of the tree."""
