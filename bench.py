@@ -73,7 +73,8 @@ def bench_lde(log_n, cols, log_blowup, reps, warmup):
 
 def bench_ragged(count, log_lo, log_hi, reps, warmup, plain=True):
     """Ragged batch: the persistent work-queue kernel over every job (one
-    launch per dependency level) vs the plain path (separate launch sequences
+    cooperative launch, workgroups waiting for each other between dependency
+    levels) vs the plain path (separate launch sequences
     per job, on the same stream).  Both must give
     the same bits.  Algorithmic bytes: 16 n (NTT/iNTT), 8 (1 + B) n (LDE)."""
     import torch
@@ -103,6 +104,7 @@ def bench_ragged(count, log_lo, log_hi, reps, warmup, plain=True):
         return sorted(ev[2 * r].elapsed_time(ev[2 * r + 1]) for r in range(reps))[reps // 2]
 
     ms_ragged = timed(plan.run)
+    plan.wait_status()  # raises if a wait between levels gave up (outputs invalid)
     ragged_out = [j.outputs.clone() for j in jobs]
     ms_plain = timed(lambda: ragged.run_plain(jobs)) if plain else float("nan")
     same = all(torch.equal(a, j.outputs) for a, j in zip(ragged_out, jobs)) if plain else None
@@ -111,7 +113,7 @@ def bench_ragged(count, log_lo, log_hi, reps, warmup, plain=True):
     torch.cuda.empty_cache()
     gbs = algo / (ms_ragged * 1e-3) / 1e9
     return {"workload": f"ragged batch: {count} single-column jobs, log_n ~ U[{log_lo},{log_hi}], "
-                        f"op ~ U{{NTT, iNTT, LDE x2, LDE x4}}, persistent queue kernel",
+                        f"op ~ U{{NTT, iNTT, LDE x2, LDE x4}}, one persistent launch of the queue kernel",
             "ms_ragged": ms_ragged, "ms_plain": ms_plain, "speedup_vs_plain": ms_plain / ms_ragged,
             "input_elems": in_elems, "output_elems": out_elems,
             "output_elems_per_s": out_elems / (ms_ragged * 1e-3),

@@ -74,6 +74,11 @@ class RaggedPlan:
         s = (stream or torch.cuda.current_stream()).cuda_stream
         _check(lib().bc_ragged_run(self._h, s), "ragged_run")
 
+    def wait_status(self):
+        """After the last run's stream has been synchronized: raises HipError
+        (BC_ERROR_WAIT_TIMEOUT) if a wait between levels gave up."""
+        _check(lib().bc_ragged_wait_status(self._h), "ragged_run (wait between levels)")
+
     def info(self):
         u, lv, wg = ctypes.c_uint64(), ctypes.c_uint32(), ctypes.c_uint32()
         _check(lib().bc_ragged_plan_info(self._h, ctypes.byref(u), ctypes.byref(lv), ctypes.byref(wg)), "plan_info")
@@ -98,3 +103,14 @@ def run_no_alloc(jobs, stream=None):
     import torch
     s = (stream or torch.cuda.current_stream()).cuda_stream
     _check(lib().bc_ragged_run_no_alloc(_array(jobs), len(jobs), s), "ragged_run_no_alloc")
+
+
+def no_alloc_wait_status():
+    """As RaggedPlan.wait_status, for the last run_no_alloc on this device."""
+    _check(lib().bc_ragged_no_alloc_wait_status(), "ragged_run_no_alloc (wait between levels)")
+
+
+def set_wait_limit(spins):
+    """Polls of one wait between levels before it gives up (0: default); for
+    tests of the timeout path."""
+    _check(lib().bc_ragged_set_wait_limit(int(spins)), "ragged_set_wait_limit")

@@ -21,9 +21,11 @@
 #include "ntt_ragged.hpp"
 #include "ntt_passes.hpp"
 
+#include <stddef.h>
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
 #include <map>
 #include <mutex>
 #include <vector>
@@ -115,12 +117,33 @@ __device__ __forceinline__ void run_unit(const RaggedSeg *sg, uint32_t variant, 
 
 struct QueueState {
   unsigned long long head[RAGGED_MAX_LEVELS];  // units popped so far, per level
+  unsigned arrive[RAGGED_MAX_LEVELS];          // single launch: workgroups done with each level
+  unsigned timed_out;                          // single launch: a level wait hit its spin limit
 };
 
+// Clears the queue state with ordinary stores (a fill kernel, so a captured
+// launch sequence holds no memset node).
 __global__ void ragged_reset_kernel(QueueState *st) {
   const uint32_t i = threadIdx.x;
-  if (i < RAGGED_MAX_LEVELS) st->head[i] = 0;
+  if (i < RAGGED_MAX_LEVELS) {
+    st->head[i] = 0;
+    st->arrive[i] = 0;
+  }
+  if (i == 0) st->timed_out = 0;
 }
+
+// Level ranges of a plan, passed by value to the single-launch kernel.
+struct RaggedLevels {
+  uint64_t item[RAGGED_MAX_LEVELS + 1];
+  uint32_t seg[RAGGED_MAX_LEVELS + 1];
+  uint32_t count;
+};
+
+// Polls of one level wait before it gives up: each is a 512-clock sleep and
+// an uncached load, about 1-2 us, so 2^22 polls are seconds, against at most
+// tens of milliseconds for a level.  A host call can lower it to force the
+// timeout path.
+constexpr uint32_t RAGGED_DEFAULT_SPIN_LIMIT = 1u << 22;
 
 // One launch drains the units of one dependency level: segments
 // [seg_begin, seg_end) hold queue items [item_begin, item_end), and no unit of
@@ -133,11 +156,10 @@ __global__ void ragged_reset_kernel(QueueState *st) {
 // `threadIdx.x == 0` blocks across the back edge, which leaves lane 0 alone on
 // the path to the pop and the rest of the workgroup in an inner loop that
 // re-reads the old unit index forever.
-__global__ __launch_bounds__(NT, 2) void ntt_ragged_kernel(const RaggedSeg *__restrict__ segs,
-                                                        const uint32_t *__restrict__ chunk_seg, uint32_t seg_begin,
-                                                        uint32_t seg_end, uint64_t item_begin, uint64_t item_end,
-                                                        unsigned long long *head) {
-  __shared__ unsigned long long s_item;
+__device__ __forceinline__ void drain_level(const RaggedSeg *__restrict__ segs,
+                                            const uint32_t *__restrict__ chunk_seg, uint32_t seg_begin,
+                                            uint32_t seg_end, uint64_t item_begin, uint64_t item_end,
+                                            unsigned long long *head, unsigned long long &s_item) {
   if (threadIdx.x == 0) s_item = item_begin + atomicAdd(head, 1ull);
   __syncthreads();
   for (;;) {
@@ -163,6 +185,61 @@ __global__ __launch_bounds__(NT, 2) void ntt_ragged_kernel(const RaggedSeg *__re
   }
 }
 
+__global__ __launch_bounds__(NT, 2) void ntt_ragged_kernel(const RaggedSeg *__restrict__ segs,
+                                                        const uint32_t *__restrict__ chunk_seg, uint32_t seg_begin,
+                                                        uint32_t seg_end, uint64_t item_begin, uint64_t item_end,
+                                                        unsigned long long *head) {
+  __shared__ unsigned long long s_item;
+  drain_level(segs, chunk_seg, seg_begin, seg_end, item_begin, item_end, head, s_item);
+}
+
+// Waits until every workgroup of the grid has arrived at this level's
+// counter.  Every wave first waits for its own memory operations, so the
+// workgroup's stores are in its XCD's L2.  Then one wave releases them
+// device-wide (an agent-scope release fence writes that L2 back), arrives,
+// polls the counter with relaxed agent-scope loads (coherent, no cache
+// maintenance), and after the last arrival takes one acquire fence, which
+// invalidates the CU's L1 and the XCD's L2 for the whole workgroup before any
+// wave reads what other workgroups wrote.  Measured on 512 workgroups: fences
+// in every wave cost 0.4 ms per level, and acquire polls (an L2 invalidation
+// each, under the workgroups still working) 0.24 ms.  The spin is bounded:
+// past spin_limit polls the wait records a timeout and returns, so every wave
+// reaches the end of the kernel.
+__device__ __forceinline__ void grid_wait(unsigned *arrive, unsigned *timed_out, uint32_t spin_limit) {
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  if (threadIdx.x < 64) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    if (threadIdx.x == 0) {
+      __hip_atomic_fetch_add(arrive, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      uint32_t spins = 0;
+      while (__hip_atomic_load(arrive, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < gridDim.x) {
+        if (++spins > spin_limit) {
+          __hip_atomic_store(timed_out, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(8);
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  }
+  __syncthreads();
+}
+
+// The whole plan in one launch (cooperative, so that every workgroup is
+// resident): the workgroups drain level 0's units, wait for each other, drain
+// level 1's, and so on.  A level's units depend only on earlier levels, so
+// the wait between levels is the only synchronization.
+__global__ __launch_bounds__(NT, 2) void ntt_ragged_all_kernel(const RaggedSeg *__restrict__ segs,
+                                                            const uint32_t *__restrict__ chunk_seg, RaggedLevels lv,
+                                                            QueueState *st, uint32_t spin_limit) {
+  __shared__ unsigned long long s_item;
+  for (uint32_t l = 0; l < lv.count; ++l) {
+    drain_level(segs, chunk_seg, lv.seg[l], lv.seg[l + 1], lv.item[l], lv.item[l + 1], &st->head[l], s_item);
+    if (l + 1 < lv.count) grid_wait(&st->arrive[l], &st->timed_out, spin_limit);
+  }
+}
+
 }  // namespace
 
 Tables current_tables(bool &ok);  // ntt_api.hip
@@ -170,20 +247,54 @@ Tables current_tables(bool &ok);  // ntt_api.hip
 size_t ragged_state_bytes() { return sizeof(QueueState); }
 
 uint32_t ragged_workgroups(int dev) {
-  int cus = 0, per_cu = 0;
+  int cus = 0, per_cu = 0, per_cu_all = 0;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) cus = 256;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, ntt_ragged_kernel, NT, 0) != hipSuccess || per_cu <= 0)
     per_cu = 1;
+  // the single-launch kernel needs the whole grid resident at once
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_all, ntt_ragged_all_kernel, NT, 0) == hipSuccess &&
+      per_cu_all > 0)
+    per_cu = std::min(per_cu, per_cu_all);
   return (uint32_t)(cus * per_cu);
 }
 
+static std::atomic<uint32_t> g_spin_limit{RAGGED_DEFAULT_SPIN_LIMIT};
+
+size_t ragged_timeout_offset() { return offsetof(QueueState, timed_out); }
+
+int ragged_read_wait_status(const void *d_state) {
+  unsigned flag = 0;
+  const hipError_t err =
+      hipMemcpy(&flag, (const uint8_t *)d_state + ragged_timeout_offset(), sizeof(flag), hipMemcpyDeviceToHost);
+  if (err != hipSuccess) return err;
+  return flag ? BC_ERROR_WAIT_TIMEOUT : BC_SUCCESS;
+}
+
 hipError_t ragged_launch(const void *d_table, const RaggedHostPlan &hp, void *d_state, uint32_t workgroups,
-                         hipStream_t s) {
+                         hipStream_t s, bool single_launch) {
   QueueState *st = (QueueState *)d_state;
   const RaggedSeg *d_segs = (const RaggedSeg *)d_table;
   const uint32_t *d_chunks = (const uint32_t *)((const uint8_t *)d_table + ragged_chunk_offset(hp));
   ragged_reset_kernel<<<dim3(1), dim3(64), 0, s>>>(st);
   hipError_t err = hipGetLastError();
+  if (err != hipSuccess || hp.levels == 0) return err;
+  if (single_launch && hp.levels <= RAGGED_MAX_LEVELS) {
+    RaggedLevels lv{};
+    for (uint32_t l = 0; l <= hp.levels; ++l) {
+      lv.item[l] = hp.level_item[l];
+      lv.seg[l] = hp.level_seg[l];
+    }
+    lv.count = hp.levels;
+    uint32_t spin_limit = g_spin_limit.load();
+    uint32_t wgs = std::max(1u, workgroups);
+    void *args[] = {(void *)&d_segs, (void *)&d_chunks, (void *)&lv, (void *)&st, (void *)&spin_limit};
+    err = hipLaunchCooperativeKernel((const void *)ntt_ragged_all_kernel, dim3(wgs), dim3(NT), args, 0, s);
+    if (err == hipSuccess) return hipSuccess;
+    // the runtime cannot make the grid resident at once: one launch per level
+    (void)hipGetLastError();
+    if (err != hipErrorCooperativeLaunchTooLarge && err != hipErrorNotSupported) return err;
+    err = hipSuccess;
+  }
   for (uint32_t l = 0; l < hp.levels && err == hipSuccess; ++l) {
     const uint64_t units = hp.level_item[l + 1] - hp.level_item[l];
     if (units == 0) continue;
@@ -247,7 +358,17 @@ extern "C" int bc_ragged_plan_create(const bc_ragged_job *jobs, uint32_t num_job
 
 extern "C" int bc_ragged_run(bc_ragged_plan *p, void *stream) {
   if (!p) return BC_ERROR_INVALID_VALUE;
-  return bc::ragged_launch(p->d_table, p->hp, p->d_state, p->workgroups, (hipStream_t)stream);
+  return bc::ragged_launch(p->d_table, p->hp, p->d_state, p->workgroups, (hipStream_t)stream, true);
+}
+
+extern "C" int bc_ragged_wait_status(const bc_ragged_plan *p) {
+  if (!p) return BC_ERROR_INVALID_VALUE;
+  return bc::ragged_read_wait_status(p->d_state);
+}
+
+extern "C" int bc_ragged_set_wait_limit(uint32_t spins) {
+  bc::g_spin_limit.store(spins ? spins : bc::RAGGED_DEFAULT_SPIN_LIMIT);
+  return BC_SUCCESS;
 }
 
 extern "C" int bc_ragged_plan_info(const bc_ragged_plan *p, uint64_t *units, uint32_t *levels, uint32_t *workgroups) {
@@ -338,10 +459,11 @@ extern "C" int bc_ragged_run_no_alloc(const bc_ragged_job *jobs, uint32_t num_jo
   RaggedHostPlan hp;
   int rc = ragged_build(jobs, num_jobs, tab, hp);
   if (rc != BC_SUCCESS) return rc;
-  // table, then the queue state on the next 256-byte boundary
+  // the queue state at the start (where bc_ragged_no_alloc_wait_status finds
+  // it), then the table on the next 256-byte boundary
   const size_t table_bytes = ragged_table_bytes(hp);
-  const size_t state_off = (table_bytes + 255) & ~(size_t)255;
-  if (state_off + ragged_state_bytes() > NOALLOC_SCRATCH_BYTES) return BC_ERROR_OUT_OF_MEMORY;
+  const size_t table_off = (ragged_state_bytes() + 255) & ~(size_t)255;
+  if (table_off + table_bytes > NOALLOC_SCRATCH_BYTES) return BC_ERROR_OUT_OF_MEMORY;
   int dev = 0;
   hipError_t err = hipGetDevice(&dev);
   if (err != hipSuccess) return err;
@@ -352,9 +474,21 @@ extern "C" int bc_ragged_run_no_alloc(const bc_ragged_job *jobs, uint32_t num_jo
   // The staging copy is pageable host memory, so HIP performs the upload
   // synchronously: the vector may be released when the call returns.
   std::vector<uint64_t> packed((table_bytes + 7) / 8);
-  ragged_pack_table(hp, packed.data(), base);
+  uint8_t *table = base + table_off;
+  ragged_pack_table(hp, packed.data(), table);
   const hipStream_t s = (hipStream_t)stream;
-  if (table_bytes) err = hipMemcpyAsync(base, packed.data(), table_bytes, hipMemcpyHostToDevice, s);
+  if (table_bytes) err = hipMemcpyAsync(table, packed.data(), table_bytes, hipMemcpyHostToDevice, s);
   if (err != hipSuccess) return err;
-  return ragged_launch(base, hp, base + state_off, wgs, s);
+  return ragged_launch(table, hp, base, wgs, s, true);
+}
+
+extern "C" int bc_ragged_no_alloc_wait_status(void) {
+  int dev = 0;
+  hipError_t err = hipGetDevice(&dev);
+  if (err != hipSuccess) return err;
+  uint8_t *base = nullptr;
+  uint32_t wgs = 0;
+  const int rc = noalloc_scratch(dev, &base, &wgs);
+  if (rc != BC_SUCCESS) return rc;
+  return bc::ragged_read_wait_status(base);
 }

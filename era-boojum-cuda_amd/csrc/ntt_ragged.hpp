@@ -58,10 +58,16 @@ size_t ragged_table_bytes(const RaggedHostPlan &hp);
 void ragged_pack_table(const RaggedHostPlan &hp, void *host_buf, const void *device_buf);
 // Byte offset of the chunk index inside the packed table.
 size_t ragged_chunk_offset(const RaggedHostPlan &hp);
-// Resets the queue state and launches the persistent kernel once per
-// dependency level on `s`; the packed table must already be in device memory
-// at d_table.  Allocates nothing.
+// Resets the queue state and launches the persistent kernel on `s`: with
+// single_launch, once for the whole plan (a cooperative launch whose
+// workgroups wait for each other between levels; if the runtime cannot make
+// the grid resident at once, it falls back to) once per dependency level.
+// The packed table must already be in device memory at d_table.  Allocates
+// nothing.
 hipError_t ragged_launch(const void *d_table, const RaggedHostPlan &hp, void *d_state, uint32_t workgroups,
-                         hipStream_t s);
+                         hipStream_t s, bool single_launch);
+// BC_ERROR_WAIT_TIMEOUT if a single launch's wait between levels gave up
+// (results invalid), else BC_SUCCESS; reads the flag with a blocking copy.
+int ragged_read_wait_status(const void *d_state);
 
 }  // namespace bc

@@ -262,7 +262,9 @@ bool bc_server::launch_batch(bc::Slot &sl) {
                         p.req->r.seed, p.first_col, sl.stream);
   }
   if (err == hipSuccess)
-    err = ragged_launch(sl.d_table, hp, sl.d_state, workgroups, sl.stream);
+    // one launch per level: batches overlap on the server's streams, which a
+    // cooperative launch (the whole grid resident at once) would serialize
+    err = ragged_launch(sl.d_table, hp, sl.d_state, workgroups, sl.stream, false);
   if (err == hipSuccess) err = hipEventRecord(sl.done, sl.stream);
   if (err != hipSuccess) {
     (void)hipStreamSynchronize(sl.stream);

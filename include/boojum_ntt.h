@@ -36,6 +36,7 @@ extern "C" {
 #define BC_ERROR_INVALID_VALUE 1   /* == hipErrorInvalidValue */
 #define BC_ERROR_OUT_OF_MEMORY 2   /* == hipErrorOutOfMemory */
 #define BC_ERROR_NOT_INITIALIZED 3 /* == hipErrorNotInitialized: no context */
+#define BC_ERROR_WAIT_TIMEOUT 702  /* == hipErrorLaunchTimeOut: a ragged run's wait between levels gave up */
 
 typedef struct bc_context bc_context;
 
@@ -110,8 +111,10 @@ int bc_graph_launch(bc_graph *graph, void *stream);
 int bc_graph_destroy(bc_graph *graph);
 
 /* Ragged batches: many independent jobs of different sizes and operations,
- * run by one persistent kernel that pulls work units from a device queue
- * (launched once per dependency level: pass k of every job together).
+ * run by one persistent kernel that pulls work units from a device queue.
+ * Units of dependency level k are pass k of every job; bc_ragged_run runs all
+ * levels in one cooperative launch whose workgroups wait for each other
+ * between levels (the serving loop launches once per level instead).
  * Each job transforms num_cols columns of 2^log_n points (8 <= log_n):
  *   BC_OP_NTT   forward, natural -> bit-reversed (bc_batch_ntt_internal with
  *               bitrev_inputs = false, inverse = false)
@@ -143,9 +146,21 @@ typedef struct bc_ragged_plan bc_ragged_plan;
 /* Plans the jobs into work units (host + one device upload).  The plan keeps
  * the job pointers; buffers must stay valid while it is run. */
 int bc_ragged_plan_create(const bc_ragged_job *jobs, uint32_t num_jobs, bc_ragged_plan **out);
-/* Resets the queue and runs every job on stream (one persistent launch per
- * dependency level, at most 16). */
+/* Resets the queue and runs every job on stream in one persistent launch
+ * (cooperative: every workgroup resident; if the runtime refuses that grid,
+ * one launch per dependency level, at most 16, with the same results). */
 int bc_ragged_run(bc_ragged_plan *plan, void *stream);
+/* After the stream of the plan's last run has been synchronized:
+ * BC_ERROR_WAIT_TIMEOUT if a wait between levels of that run gave up after
+ * its spin limit (the outputs are then invalid), else BC_SUCCESS.  The waits
+ * are bounded so that the kernel always ends.  No reference counterpart. */
+int bc_ragged_wait_status(const bc_ragged_plan *plan);
+/* As bc_ragged_wait_status, for the last bc_ragged_run_no_alloc on the
+ * current device. */
+int bc_ragged_no_alloc_wait_status(void);
+/* Polls of one wait between levels before it gives up (about 1-2 us each);
+ * 0 restores the default, 2^22.  For tests of the timeout path. */
+int bc_ragged_set_wait_limit(uint32_t spins);
 int bc_ragged_plan_destroy(bc_ragged_plan *plan);
 /* Statistics of a plan: work units, dependency levels, persistent workgroups. */
 int bc_ragged_plan_info(const bc_ragged_plan *plan, uint64_t *units, uint32_t *levels, uint32_t *workgroups);
@@ -160,8 +175,8 @@ int bc_ragged_run_plain(const bc_ragged_job *jobs, uint32_t num_jobs, void *stre
  * the process ends; a plan that needs more returns BC_ERROR_OUT_OF_MEMORY
  * before anything is launched (the 1000-job ragged benchmark batch takes
  * about 0.12 MiB).  The table is uploaded from pageable host memory, which HIP
- * copies synchronously, then the levels are launched on `stream` exactly as
- * bc_ragged_run does, with the same results.  Every call on one GPU shares
+ * copies synchronously, then the plan is launched on `stream` exactly as
+ * bc_ragged_run does (one launch), with the same results.  Every call on one GPU shares
  * that GPU's array and overwrites it: calls that overlap on one GPU (other
  * threads, or work still queued on another stream) race.  Host planning uses
  * the process heap. */

@@ -110,8 +110,9 @@ def test_config_ragged_1000_jobs(gpu_ctx):
     assert len(jobs) == 1000 and max(s[0] for s in specs) == 23 and min(s[0] for s in specs) == 10
     plan = ragged.RaggedPlan(jobs)
     try:
-        plan.run()
+        plan.run()  # one persistent launch for the whole batch
         torch.cuda.synchronize()
+        plan.wait_status()
         mixed = [j.outputs.clone() for j in jobs]
     finally:
         plan.destroy()

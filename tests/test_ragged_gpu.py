@@ -102,10 +102,44 @@ def test_ragged_mixed_batch_matches_oracle(gpu_ctx):
         assert info["levels"] >= 3 and info["units"] > 0
         plan.run()
         check(jobs, hosts)
+        plan.wait_status()
         # a second run of the same plan (counters reset on the device)
         for j in jobs:
             j.outputs.zero_()
         plan.run()
+        check(jobs, hosts)
+        plan.wait_status()
+    finally:
+        plan.destroy()
+
+
+def test_ragged_wait_timeout_is_reported(gpu_ctx):
+    """The single launch's waits between levels are bounded.  A wait limit of
+    one poll (set from the host) makes the early workgroups give up: the run
+    still ends, its status reports the timeout, and with the default limit
+    the next run of the same plan is exact and reports success."""
+    import torch
+    from boojum_cuda_amd import ragged
+    from boojum_cuda_amd._lib import HipError
+    specs = [(ragged.OP_LDE, 16, 2, 2), (ragged.OP_NTT, 20, 1, 0), (ragged.OP_INTT, 18, 3, 0)]
+    jobs, hosts = make_jobs(specs, 900)
+    plan = ragged.RaggedPlan(jobs)
+    try:
+        assert plan.info()["levels"] >= 3
+        ragged.set_wait_limit(1)
+        try:
+            plan.run()
+            torch.cuda.synchronize()
+        finally:
+            ragged.set_wait_limit(0)
+        with pytest.raises(HipError) as e:
+            plan.wait_status()
+        assert e.value.code == 702
+        for j in jobs:
+            j.outputs.zero_()
+        plan.run()
+        torch.cuda.synchronize()
+        plan.wait_status()
         check(jobs, hosts)
     finally:
         plan.destroy()
