@@ -35,7 +35,6 @@ SIGNATURES = {
     "bc_ragged_run_plain": (c_int, [c_vp, c_u32, c_vp]),
     "bc_ragged_run_no_alloc": (c_int, [c_vp, c_u32, c_vp]),
     "bc_ragged_wait_status": (c_int, [c_vp]),
-    "bc_ragged_no_alloc_wait_status": (c_int, []),
     "bc_ragged_set_wait_limit": (c_int, [c_u32]),
     "bc_ntt_describe_plan": (c_int, [c_u32, c_u32, c_bool, c_bool, c_u32, c_u32, c_vp, c_u32,
                                      ctypes.POINTER(c_u32)]),

@@ -76,7 +76,7 @@ class RaggedPlan:
 
     def wait_status(self):
         """After the last run's stream has been synchronized: raises HipError
-        (BC_ERROR_WAIT_TIMEOUT) if a wait between levels gave up."""
+        (BC_ERROR_WAIT_TIMEOUT) if a wait between levels of that run gave up."""
         _check(lib().bc_ragged_wait_status(self._h), "ragged_run (wait between levels)")
 
     def info(self):
@@ -103,11 +103,6 @@ def run_no_alloc(jobs, stream=None):
     import torch
     s = (stream or torch.cuda.current_stream()).cuda_stream
     _check(lib().bc_ragged_run_no_alloc(_array(jobs), len(jobs), s), "ragged_run_no_alloc")
-
-
-def no_alloc_wait_status():
-    """As RaggedPlan.wait_status, for the last run_no_alloc on this device."""
-    _check(lib().bc_ragged_no_alloc_wait_status(), "ragged_run_no_alloc (wait between levels)")
 
 
 def set_wait_limit(spins):

@@ -278,6 +278,14 @@ hipError_t ragged_launch(const void *d_table, const RaggedHostPlan &hp, void *d_
   ragged_reset_kernel<<<dim3(1), dim3(64), 0, s>>>(st);
   hipError_t err = hipGetLastError();
   if (err != hipSuccess || hp.levels == 0) return err;
+  // a stream under graph capture gets plain launches (one per level)
+  if (single_launch) {
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(s, &cap) != hipSuccess || cap != hipStreamCaptureStatusNone) {
+      (void)hipGetLastError();
+      single_launch = false;
+    }
+  }
   if (single_launch && hp.levels <= RAGGED_MAX_LEVELS) {
     RaggedLevels lv{};
     for (uint32_t l = 0; l <= hp.levels; ++l) {
@@ -459,8 +467,7 @@ extern "C" int bc_ragged_run_no_alloc(const bc_ragged_job *jobs, uint32_t num_jo
   RaggedHostPlan hp;
   int rc = ragged_build(jobs, num_jobs, tab, hp);
   if (rc != BC_SUCCESS) return rc;
-  // the queue state at the start (where bc_ragged_no_alloc_wait_status finds
-  // it), then the table on the next 256-byte boundary
+  // the queue state at the start, then the table on the next 256-byte boundary
   const size_t table_bytes = ragged_table_bytes(hp);
   const size_t table_off = (ragged_state_bytes() + 255) & ~(size_t)255;
   if (table_off + table_bytes > NOALLOC_SCRATCH_BYTES) return BC_ERROR_OUT_OF_MEMORY;
@@ -479,16 +486,6 @@ extern "C" int bc_ragged_run_no_alloc(const bc_ragged_job *jobs, uint32_t num_jo
   const hipStream_t s = (hipStream_t)stream;
   if (table_bytes) err = hipMemcpyAsync(table, packed.data(), table_bytes, hipMemcpyHostToDevice, s);
   if (err != hipSuccess) return err;
-  return ragged_launch(table, hp, base, wgs, s, true);
-}
-
-extern "C" int bc_ragged_no_alloc_wait_status(void) {
-  int dev = 0;
-  hipError_t err = hipGetDevice(&dev);
-  if (err != hipSuccess) return err;
-  uint8_t *base = nullptr;
-  uint32_t wgs = 0;
-  const int rc = noalloc_scratch(dev, &base, &wgs);
-  if (rc != BC_SUCCESS) return rc;
-  return bc::ragged_read_wait_status(base);
+  // plain kernel launches, one per level (no cooperative launch)
+  return ragged_launch(table, hp, base, wgs, s, false);
 }

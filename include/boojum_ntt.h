@@ -147,17 +147,15 @@ typedef struct bc_ragged_plan bc_ragged_plan;
  * the job pointers; buffers must stay valid while it is run. */
 int bc_ragged_plan_create(const bc_ragged_job *jobs, uint32_t num_jobs, bc_ragged_plan **out);
 /* Resets the queue and runs every job on stream in one persistent launch
- * (cooperative: every workgroup resident; if the runtime refuses that grid,
- * one launch per dependency level, at most 16, with the same results). */
+ * (cooperative: every workgroup resident).  On a stream under graph capture,
+ * or if the runtime refuses that grid, it launches once per dependency level
+ * instead (at most 16), with the same results. */
 int bc_ragged_run(bc_ragged_plan *plan, void *stream);
 /* After the stream of the plan's last run has been synchronized:
  * BC_ERROR_WAIT_TIMEOUT if a wait between levels of that run gave up after
  * its spin limit (the outputs are then invalid), else BC_SUCCESS.  The waits
  * are bounded so that the kernel always ends.  No reference counterpart. */
 int bc_ragged_wait_status(const bc_ragged_plan *plan);
-/* As bc_ragged_wait_status, for the last bc_ragged_run_no_alloc on the
- * current device. */
-int bc_ragged_no_alloc_wait_status(void);
 /* Polls of one wait between levels before it gives up (about 1-2 us each);
  * 0 restores the default, 2^22.  For tests of the timeout path. */
 int bc_ragged_set_wait_limit(uint32_t spins);
@@ -175,8 +173,9 @@ int bc_ragged_run_plain(const bc_ragged_job *jobs, uint32_t num_jobs, void *stre
  * the process ends; a plan that needs more returns BC_ERROR_OUT_OF_MEMORY
  * before anything is launched (the 1000-job ragged benchmark batch takes
  * about 0.12 MiB).  The table is uploaded from pageable host memory, which HIP
- * copies synchronously, then the plan is launched on `stream` exactly as
- * bc_ragged_run does (one launch), with the same results.  Every call on one GPU shares
+ * copies synchronously, then the plan is launched on `stream` once per
+ * dependency level (plain kernel launches, no cooperative launch), with
+ * bc_ragged_run's results.  Every call on one GPU shares
  * that GPU's array and overwrites it: calls that overlap on one GPU (other
  * threads, or work still queued on another stream) race.  Host planning uses
  * the process heap. */

@@ -145,6 +145,45 @@ def test_ragged_wait_timeout_is_reported(gpu_ctx):
         plan.destroy()
 
 
+def test_ragged_run_in_a_graph_replays_exact(gpu_ctx):
+    """bc_ragged_run on a stream under graph capture launches once per level
+    (the single cooperative launch is not captured): every replay, with eager
+    work of other shapes between replays, gives the eager run's bits."""
+    import torch
+    from boojum_cuda_amd import ragged
+    specs = specs_mixed(3, 10, 9, 16) + [(ragged.OP_LDE, 15, 2, 2)]
+    jobs, hosts = make_jobs(specs, 300)
+    plan = ragged.RaggedPlan(jobs)
+    other_jobs, _ = make_jobs(specs_mixed(4, 6, 8, 13), 400)
+    other = ragged.RaggedPlan(other_jobs)
+    try:
+        plan.run()
+        torch.cuda.synchronize()
+        plan.wait_status()
+        want = [to_host(j.outputs).copy() for j in jobs]
+        graph = torch.cuda.CUDAGraph()
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(side):
+            with torch.cuda.graph(graph, stream=side):
+                plan.run(stream=side)
+        torch.cuda.current_stream().wait_stream(side)
+        torch.cuda.synchronize()
+        for k in range(3):
+            for j in jobs:
+                j.outputs.zero_()
+            graph.replay()
+            torch.cuda.synchronize()
+            for j, w in zip(jobs, want):
+                assert np.array_equal(to_host(j.outputs), w), k
+            other.run()  # eager work between replays
+            torch.cuda.synchronize()
+        check(jobs, hosts)
+    finally:
+        other.destroy()
+        plan.destroy()
+
+
 def test_ragged_equals_plain_and_alone(gpu_ctx):
     from boojum_cuda_amd import ragged
     specs = specs_mixed(7, 40, 10, 18)

@@ -42,7 +42,6 @@ def _run_no_alloc_abi(jobs):
     rc = lib().bc_ragged_run_no_alloc(arr, len(jobs), ctypes.c_void_p(torch.cuda.current_stream().cuda_stream))
     assert rc == 0, rc
     torch.cuda.synchronize()
-    assert lib().bc_ragged_no_alloc_wait_status() == 0  # no wait between levels gave up
 
 
 def test_no_alloc_matches_planned_run_twice(gpu_ctx):
