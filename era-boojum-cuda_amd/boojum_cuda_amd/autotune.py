@@ -281,6 +281,19 @@ def _range(text):
     return range(int(lo), int(hi or lo) + 1)
 
 
+def kept_entries(path, tuned):
+    """Entries of the database at `path` outside the (log_n, log_cols)
+    buckets in `tuned`, as (log_n, log_cols, TuneParams): what --merge-from
+    carries over (comment lines skipped)."""
+    out = []
+    for line in open(path):
+        f = line.split()
+        if not f or f[0].startswith("#") or (int(f[0]), int(f[1])) in tuned:
+            continue
+        out.append((int(f[0]), int(f[1]), TuneParams(*[int(v) for v in f[2:]])))
+    return out
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--log-n", default="12-24")
@@ -288,7 +301,7 @@ def main(argv=None):
     ap.add_argument("--reps", type=int, default=5)
     ap.add_argument("--out", required=True)
     ap.add_argument("--merge-from", default=None,
-                    help="database whose entries for the log_n values not tuned here are kept in --out")
+                    help="database whose entries for the (log_n, log_cols) buckets not tuned here are kept in --out")
     args = ap.parse_args(argv)
     import torch
     from .context import Context
@@ -301,12 +314,9 @@ def main(argv=None):
             tune_shape(log_n, log_cols, args.reps, log=lambda m: print(m, flush=True))
         torch.cuda.empty_cache()
     if args.merge_from:
-        tuned_ns = set(_range(args.log_n))
-        for line in open(args.merge_from):
-            f = line.split()
-            if not f or f[0].startswith("#") or int(f[0]) in tuned_ns:
-                continue
-            set_entry(int(f[0]), int(f[1]), TuneParams(*[int(v) for v in f[2:]]))
+        tuned = {(n, c) for n in _range(args.log_n) for c in _range(args.log_cols)}
+        for log_n, log_cols, params in kept_entries(args.merge_from, tuned):
+            set_entry(log_n, log_cols, params)
     os.makedirs(os.path.dirname(os.path.abspath(args.out)), exist_ok=True)
     save(args.out)
     print(f"saved {args.out} in {time.time() - t0:.0f} s", flush=True)

@@ -105,3 +105,18 @@ def test_builtin_split_is_a_valid_split_and_matches_the_planner_rule():
         assert autotune.builtin_split(log_n) == split, log_n
     for log_n in range(13, 33):
         assert autotune.builtin_split(log_n) in autotune.splits(log_n), log_n
+
+
+def test_merge_keeps_every_bucket_not_tuned(tmp_path):
+    """--merge-from carries over every entry outside the tuned
+    (log_n, log_cols) buckets, including other column buckets of a tuned
+    log_n (host-only: reads the file, no device)."""
+    from boojum_cuda_amd import autotune
+    db = tmp_path / "db.txt"
+    db.write_text("# header\n"
+                  "16 0 0 0 0 0 4 4 8\n"
+                  "16 3 0 0 2 0 4 4 0\n"
+                  "20 0 0 0 1 0 0 0 0\n"
+                  "20 8 9 2 0 0 0 0 0\n")
+    kept = autotune.kept_entries(str(db), {(20, 0), (16, 3)})
+    assert [(n, c, p.key()) for n, c, p in kept] == [(16, 0, (0, 0, 0, 0, 4, 4, 8)), (20, 8, (9, 2, 0, 0, 0, 0, 0))]
