@@ -170,3 +170,13 @@ def test_plan_of_the_latency_path():
             assert plan[0 if br else 1]["contiguous"] == 1
     assert all(p["workgroup_threads"] == 256 for p in ntt.describe_plan(16, 5))
     assert all(p["workgroup_threads"] == 256 for p in ntt.describe_plan(16, 1, log_extension_degree=1))
+
+
+def test_ragged_wait_calls_check_arguments_without_a_device():
+    """The ragged wait-status and wait-limit calls are host-side: a null plan
+    is an argument error, and the limit setter accepts any value (0 restores
+    the default)."""
+    from boojum_cuda_amd._lib import lib
+    assert lib().bc_ragged_wait_status(None) == 1  # BC_ERROR_INVALID_VALUE
+    assert lib().bc_ragged_set_wait_limit(5) == 0
+    assert lib().bc_ragged_set_wait_limit(0) == 0
