@@ -145,10 +145,11 @@ struct RaggedLevels {
 // timeout path.
 constexpr uint32_t RAGGED_DEFAULT_SPIN_LIMIT = 1u << 22;
 
-// One launch drains the units of one dependency level: segments
-// [seg_begin, seg_end) hold queue items [item_begin, item_end), and no unit of
-// a level depends on another unit of the same level, so units need no waits
-// and no device-scope fences; the launch boundary orders the levels.
+// Drains the units of one dependency level: segments [seg_begin, seg_end)
+// hold queue items [item_begin, item_end), and no unit of a level depends on
+// another unit of the same level, so units need no waits and no device-scope
+// fences.  Between levels, the launch boundary (one launch per level) or
+// grid_wait (the single launch) orders them.
 //
 // Loop shape: the only single-thread block that touches the queue sits at the
 // end of the body and is followed by a barrier in uniform control flow.  With
