@@ -94,6 +94,13 @@ __device__ __forceinline__ uint64_t mad_carry(uint32_t a, uint32_t b, uint64_t c
   return r;
 }
 
+// mad_carry with b in an SGPR (b the same in every lane).
+__device__ __forceinline__ uint64_t mad_carry_s(uint32_t a, uint32_t b, uint64_t c, uint64_t &carry) {
+  uint64_t r;
+  asm("v_mad_u64_u32 %0, %1, %2, %3, %4" : "=v"(r), "=s"(carry) : "v"(a), "s"(b), "v"(c));
+  return r;
+}
+
 // z - h - cin (cin a lane mask) with the final borrow in `borrow`.
 __device__ __forceinline__ uint64_t sub32_borrow(uint64_t z, uint32_t h, uint64_t cin, uint64_t &borrow) {
   uint32_t lo, hi;
@@ -193,6 +200,27 @@ GL_FN uint64_t mul(uint64_t a, uint64_t b) {
   const uint32_t lo = __builtin_subc((uint32_t)z, (uint32_t)(t3 >> 32), (unsigned)c, &b0w);
   const uint32_t hi = __builtin_subc((uint32_t)(z >> 32), 0u, b0w, &bw);
   return fold(((uint64_t)hi << 32) | lo, c2 && !bw, bw && !c2);
+#endif
+}
+
+// mul for a b that is the same in every lane of the wave (the caller's
+// promise: lanes of another b get lane 0's).  On gfx950 b is taken into SGPRs,
+// so a loop-invariant b hoisted out of a loop costs no VGPRs.
+GL_FN uint64_t mul_uniform(uint64_t a, uint64_t b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  const uint32_t a0 = (uint32_t)a, a1 = (uint32_t)(a >> 32);
+  const uint32_t b0 = __builtin_amdgcn_readfirstlane((uint32_t)b);
+  const uint32_t b1 = __builtin_amdgcn_readfirstlane((uint32_t)(b >> 32));
+  const uint64_t t0 = (uint64_t)a0 * b0;
+  const uint64_t t1 = (uint64_t)a0 * b1 + (t0 >> 32);
+  uint64_t c, c2, bw;
+  const uint64_t t2 = dev::mad_carry_s(a1, b0, t1, c);
+  const uint64_t t3 = (uint64_t)a1 * b1 + (t2 >> 32);
+  const uint64_t z = dev::mad_carry((uint32_t)t3, 0xFFFFFFFFu, (t2 << 32) | (uint32_t)t0, c2);
+  const uint64_t w = dev::sub32_borrow(z, (uint32_t)(t3 >> 32), c, bw);
+  return dev::fold_masks(w, c2, bw);
+#else
+  return mul(a, b);
 #endif
 }
 

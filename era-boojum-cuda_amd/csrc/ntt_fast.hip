@@ -20,11 +20,11 @@ __global__ __launch_bounds__(NT, DIT ? 2 : 3) void ntt_contig_kernel(FastArgs a)
   contig_run<LH, INV, DIT, MODE, MC>(a, sh, blockIdx.x, a.tiles, gridDim.x);
 }
 
-// Three workgroups per CU as well, except for the coset-scaling variants,
-// whose 16 extra factors per thread would spill at 168 VGPRs.
+// Three workgroups per CU as well (the coset-scaling variants included: their
+// factors live in the pass and level twists and a 16-entry LDS table, not in
+// per-thread registers).
 template <int LH, bool INV, bool DIT, int MODE>
-__global__ __launch_bounds__(NT, MODE == MODE_LDE_IN || MODE == MODE_LDE_OUT ? 2 : 3) void ntt_strided_kernel(
-    FastArgs a) {
+__global__ __launch_bounds__(NT, 3) void ntt_strided_kernel(FastArgs a) {
   __shared__ PassLds sh;
   const uint64_t total = (uint64_t)a.num_cols << (a.log_n - a.log_N);
   const uint64_t p_begin = (uint64_t)blockIdx.y * a.blocks_per_wg;

@@ -87,6 +87,19 @@ template <int MODE> __device__ __forceinline__ uint64_t coset_part_factor(const 
   return MODE == MODE_LDE_OUT ? gl::mul(a.n_inv, v) : v;
 }
 
+// Level twists with a coset factor folded in (strided passes): group rp's
+// twists become tw * c(rp) with c(rp) = s^(+-rp 2^log_r), and dst0[rp] = c(rp)
+// scales the group's point 0, whose twist is 1.
+template <int LM, bool INV, int MODE>
+__device__ __forceinline__ void stage_level_twists_coset(uint64_t *dst, uint64_t *dst0, const FastArgs &a,
+                                                         uint32_t log_r) {
+  constexpr uint32_t S = 1u << (LM - 4), LEN = 15u * S;
+  const uint64_t *src = lvl_table<LM, INV>(a.tab);
+  for (uint32_t k = threadIdx.x; k < LEN; k += NT)
+    dst[k] = gl::mul(src[k], coset_scale_pow<MODE>(a, (k & (S - 1u)) << log_r));
+  if (threadIdx.x < S) dst0[threadIdx.x] = coset_scale_pow<MODE>(a, threadIdx.x << log_r);
+}
+
 // Column addressing: one matrix (base + c * stride) for the standalone
 // launches, or a per-column pointer table (CT) for ragged job groups.
 template <bool CT> __device__ __forceinline__ const uint64_t *in_col(const FastArgs &a, uint64_t c) {
@@ -176,6 +189,7 @@ struct PassLds {
   uint64_t tile[LDS_PADDED];
   uint64_t tw[LDS_TW_LEN];
   uint64_t cf[16];  // coset part factors (LDE modes)
+  uint64_t cr[16];  // coset factors of the strided level-0 groups (LDE modes)
 };
 
 // Runs tiles tile_begin, tile_begin + step, ... < tile_end of a contiguous
@@ -425,29 +439,41 @@ __device__ __forceinline__ void strided_run(const FastArgs &a, PassLds &sh, uint
     else return base0;
   };
 
-  if constexpr (NL == 2) stage_level_twists<LH, INV>(tw0, a.tab);
+  // Coset scaling (this is the top pass: one block per column).  The point of
+  // level-0 slot i sits at column position i 2^(log_N - 4) + rp 2^log_r + r
+  // (r = r0 + q), and its factor s^(+-pos) splits into three parts, each
+  // folded where it costs no extra multiply per point: s^(+-r) is common to
+  // the whole sub-DFT at r and joins its pass twist; s^(+-rp 2^log_r) is
+  // common to one level-0 DFT and joins that level's twists (plus point 0);
+  // s^(+-i 2^(log_N - 4)) multiplies slot i (n2b inputs, b2n outputs) from a
+  // 16-entry LDS table.  n^-1 of the inverse coset transform joins the pass
+  // twist.
+  constexpr bool SC = MODE == MODE_LDE_IN || MODE == MODE_LDE_OUT;
+  if constexpr (NL == 2) {
+    if constexpr (SC) stage_level_twists_coset<LH, INV, MODE>(tw0, sh.cr, a, log_r);
+    else stage_level_twists<LH, INV>(tw0, a.tab);
+  }
+  if constexpr (SC) {
+    if (t < 16) sh.cf[t] = coset_scale_pow<MODE>(a, t << (a.log_N - 4));
+  }
 
   // pass twist omega_N^((r0 + q) * brev_h(j)) in the last-level mapping, with
   // n^-1 folded in when this pass carries the inverse scaling
   uint64_t pt[16];
 #pragma unroll
-  for (int g = 0; g < G1; ++g)
+  for (int g = 0; g < G1; ++g) {
+    uint64_t cq = 0;
+    if constexpr (SC) cq = coset_scale_pow<MODE>(a, r0 + lastq(g));
 #pragma unroll
     for (int i = 0; i < (1 << A1); ++i) {
       const uint32_t j = lastj(g, i);
       uint32_t e = ((r0 + lastq(g)) * brev_bits(j, LH)) << (32u - a.log_N);
       if (INV) e = 0u - e;
       uint64_t w = pow_w(a.tab, e);
-      if constexpr (MODE == MODE_NINV) w = gl::mul(w, a.n_inv);
+      if constexpr (MODE == MODE_NINV || MODE == MODE_LDE_OUT) w = gl::mul(w, a.n_inv);
+      if constexpr (SC) w = gl::mul(w, cq);
       pt[g * (1 << A1) + i] = w;
     }
-  // coset factors at column positions (this is the top pass: one block per
-  // column) i 2^(log_N - 4) + base of the level-0 mapping (n2b inputs, b2n
-  // outputs): the part factors of i go to LDS, read after the barrier below
-  constexpr bool SC = MODE == MODE_LDE_IN || MODE == MODE_LDE_OUT;
-  uint64_t cf[SC ? 16 : 1];
-  if constexpr (SC) {
-    if (t < 16) sh.cf[t] = coset_part_factor<MODE>(a, t << (a.log_N - 4));
   }
 
   const uint32_t blocks_per_col_log = a.log_n - a.log_N;
@@ -474,12 +500,7 @@ __device__ __forceinline__ void strided_run(const FastArgs &a, PassLds &sh, uint
 
   uint64_t pf[16];
   if (p_begin < p_end) load(p_begin, pf);
-  __syncthreads();  // tw0 and coset part factors staged
-  if constexpr (SC) {
-    const uint64_t cfb = coset_scale_pow<MODE>(a, ((uint32_t)rp0 << log_r) + r0 + q0);
-#pragma unroll
-    for (int i = 0; i < 16; ++i) cf[i] = gl::mul(cfb, sh.cf[i]);
-  }
+  __syncthreads();  // tw0 and coset factors staged
   for (uint64_t p = p_begin; p < p_end; ++p) {
     uint64_t x[16];
 #pragma unroll
@@ -488,8 +509,9 @@ __device__ __forceinline__ void strided_run(const FastArgs &a, PassLds &sh, uint
     uint64_t *out = out_col<CT>(a, p >> blocks_per_col_log) + pair_pos(p);
     if constexpr (!DIT) {
       if constexpr (MODE == MODE_LDE_IN) {
+        if (a.first) x[0] = gl::canon(x[0]);  // field ops take canonical values
 #pragma unroll
-        for (int i = 0; i < 16; ++i) x[i] = gl::mul(x[i], cf[i]);
+        for (int i = 1; i < 16; ++i) x[i] = gl::mul_uniform(x[i], sh.cf[i]);
       } else if (a.first) {
 #pragma unroll
         for (int i = 0; i < 16; ++i) x[i] = gl::canon(x[i]);  // field ops take canonical values
@@ -497,6 +519,7 @@ __device__ __forceinline__ void strided_run(const FastArgs &a, PassLds &sh, uint
       dft_dif<4, INV>(x);
       if constexpr (NL == 2) {
         level_twist_lds<LH>(x, tw0, rp0);
+        if constexpr (SC) x[0] = gl::mul(x[0], sh.cr[rp0]);
         // LDS row-major by j: element (q, j) at j * T + q
 #pragma unroll
         for (int i = 0; i < 16; ++i) lds[((((uint32_t)i << LS0) + rp0) << LT) + q0] = x[i];
@@ -534,13 +557,16 @@ __device__ __forceinline__ void strided_run(const FastArgs &a, PassLds &sh, uint
         for (int i = 0; i < 16; ++i) x[i] = lds[((((uint32_t)i << LS0) + rp0) << LT) + q0];
         __syncthreads();
         level_twist_lds<LH>(x, tw0, rp0);
+        if constexpr (SC) x[0] = gl::mul(x[0], sh.cr[rp0]);
       }
       dft_dit<4, INV>(x);
       uint64_t *q = out + base0;
 #pragma unroll
       for (int i = 0; i < 16; ++i, q += step0) {
         uint64_t v = x[i];
-        if constexpr (MODE == MODE_LDE_OUT) v = gl::mul(v, cf[i]);
+        if constexpr (MODE == MODE_LDE_OUT) {
+          if (i > 0) v = gl::mul_uniform(v, sh.cf[i]);
+        }
         __builtin_nontemporal_store(v, q);
       }
     }

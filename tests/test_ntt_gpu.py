@@ -108,7 +108,7 @@ def test_correctness_batch_ntt_inv_large(gpu_ctx, log_n):
     correctness([log_n], True, 0, 0, 1)
 
 
-@pytest.mark.parametrize("log_n", [17, 19, 21])
+@pytest.mark.parametrize("log_n", [17, 18, 19, 20, 21])
 @pytest.mark.parametrize("inverse", [False, True])
 def test_correctness_batch_lde_large(gpu_ctx, log_n, inverse):
     correctness([log_n], inverse, 3, 3, 3)
