@@ -22,9 +22,13 @@ __global__ __launch_bounds__(NT, DIT ? 2 : 3) void ntt_contig_kernel(FastArgs a)
 
 // Three workgroups per CU as well (the coset-scaling variants included: their
 // factors live in the pass and level twists and a 16-entry LDS table, not in
-// per-thread registers).
+// per-thread registers), except where 168 VGPRs spill heavily: h = 6 (four
+// radix-4 groups per thread in the second level; 70-76 spilled dwords at
+// three per CU, 28-38% slower on 512 x 2^18 and 8 x 2^24 n2b calls) and b2n
+// h = 5 (16-22 dwords, 3-4% slower).  The other b2n variants spill 4-8
+// dwords at three per CU and still ran faster there than at two.
 template <int LH, bool INV, bool DIT, int MODE>
-__global__ __launch_bounds__(NT, 3) void ntt_strided_kernel(FastArgs a) {
+__global__ __launch_bounds__(NT, LH == 6 || (LH == 5 && DIT) ? 2 : 3) void ntt_strided_kernel(FastArgs a) {
   __shared__ PassLds sh;
   const uint64_t total = (uint64_t)a.num_cols << (a.log_n - a.log_N);
   const uint64_t p_begin = (uint64_t)blockIdx.y * a.blocks_per_wg;
