@@ -65,6 +65,18 @@ struct PosTable {
 
 __device__ PosTable pos_table;
 
+// The table, through an address the optimizer cannot see through: a kernel
+// that permutes in a loop then reloads the constants it needs in each pass
+// (scalar-cache hits) instead of hoisting them all into more SGPRs than a
+// wave has, which spilled 490 of them to VGPR lanes in the Poseidon leaf
+// kernel (5% faster leaves without; Poseidon2, with fewer constants, keeps
+// the direct table, 1% faster that way).
+__device__ __forceinline__ const PosTable &table() {
+  const PosTable *p = &pos_table;
+  asm volatile("" : "+s"(p));
+  return *p;
+}
+
 __device__ __forceinline__ uint64_t sbox(uint64_t x) {
   const uint64_t x2 = gl::mul(x, x);
   const uint64_t x3 = gl::mul(x2, x);
@@ -100,7 +112,7 @@ __device__ __forceinline__ void full_round(uint64_t (&s)[PT], const uint64_t *rc
 }
 
 __device__ __forceinline__ void permute(uint64_t (&s)[PT]) {
-  const PosTable &t = pos_table;
+  const PosTable &t = table();
 #pragma unroll 1
   for (int r = 0; r < HALF_FULL - 1; ++r) {
     full_round(s, t.rc_full[r]);
@@ -192,7 +204,7 @@ __device__ __forceinline__ void m_int(uint64_t (&s)[PT]) {
 }
 
 __device__ __forceinline__ void permute2(uint64_t (&s)[PT]) {
-  const PosTable &t = pos_table;
+  const PosTable &t = pos_table;  // its few constants hoist without spills
   m_ext(s);
 #pragma unroll 1
   for (int r = 0; r < HALF_FULL; ++r) {
