@@ -8,12 +8,11 @@ namespace bc {
 namespace {
 using namespace pass;
 
-// Three workgroups per CU for the DIF (n2b) variants: the 12-bit pass then
-// fits in 168 VGPRs (a few loop-invariant dwords spill) and runs 3 waves per
-// SIMD instead of 2, which hides more of its LDS exchanges (6% faster on
-// 256 x 2^20).  The DIT (b2n) variants, which stage their loads through one
-// more LDS exchange, spill 15-20 VGPRs at that bound and run 1-3% faster at
-// two workgroups per CU without spills.
+// Register bounds of three workgroups per CU for the DIF (n2b) variants and
+// two for the DIT (b2n) ones.  With their LDS slots addressed as one base
+// plus immediate offsets, the 12-bit kernels need 124 (n2b) and 142 (b2n)
+// VGPRs without spills and so run 4 and 3 workgroups per CU (LDS allows 4);
+// a bound of four workgroups would spill 14-26 dwords in the b2n variants.
 template <int LH, bool INV, bool DIT, int MODE, bool MC>
 __global__ __launch_bounds__(NT, DIT ? 2 : 3) void ntt_contig_kernel(FastArgs a) {
   __shared__ PassLds sh;
@@ -87,25 +86,29 @@ hipError_t strided_dir(const FastArgs &a, bool inv, bool dit, int mode, dim3 gri
 // workgroups.  The tile loop is grid-strided, so a grid that is a whole
 // number of rounds leaves no partly filled last round (2048 workgroups on the
 // 768 slots of 256 CUs x 3 ran 2.67 rounds and were 4% slower on 256 x 2^20).
-// Queried once per device; all contiguous variants share the launch bound.
+// Queried once per device and direction: the n2b and b2n 12-bit kernels
+// differ in register use and so in workgroups per CU.
 constexpr int MAX_DEVICES = 64;
 std::mutex g_grid_mu;
-uint32_t g_grid[MAX_DEVICES] = {};
+uint32_t g_grid[MAX_DEVICES][2] = {};
 
-uint32_t default_contig_grid() {
+uint32_t default_contig_grid(bool dit) {
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= MAX_DEVICES) return 2048;
   std::lock_guard<std::mutex> lk(g_grid_mu);
-  if (!g_grid[dev]) {
+  uint32_t &g = g_grid[dev][dit ? 1 : 0];
+  if (!g) {
     int cus = 0, per_cu = 0;
     const bool ok =
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, ntt_contig_kernel<12, false, false, MODE_NONE, false>,
-                                                     NT, 0) == hipSuccess &&
+        (dit ? hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                   &per_cu, ntt_contig_kernel<12, false, true, MODE_NONE, false>, NT, 0)
+             : hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                   &per_cu, ntt_contig_kernel<12, false, false, MODE_NONE, false>, NT, 0)) == hipSuccess &&
         cus > 0 && per_cu > 0;
-    g_grid[dev] = ok ? (uint32_t)(2 * cus * per_cu) : 2048u;
+    g = ok ? (uint32_t)(2 * cus * per_cu) : 2048u;
   }
-  return g_grid[dev];
+  return g;
 }
 
 }  // namespace
@@ -113,7 +116,7 @@ uint32_t default_contig_grid() {
 hipError_t launch_contig(const FastArgs &a, int log_h, bool inverse, bool dit, int mode, uint32_t max_wgs,
                          hipStream_t stream) {
   if (a.tiles == 0) return hipSuccess;
-  if (max_wgs == 0) max_wgs = default_contig_grid();
+  if (max_wgs == 0) max_wgs = default_contig_grid(dit);
   const dim3 grid(a.tiles < max_wgs ? a.tiles : max_wgs);
   if (a.log_n < TILE_LOG) {  // single pass over whole columns, several per tile
     switch (log_h) {

@@ -211,6 +211,23 @@ __device__ __forceinline__ void contig_run(const FastArgs &a, PassLds &sh, uint3
   uint64_t *lds = sh.tile;
   uint64_t *tw1 = sh.tw;
   const uint32_t t = threadIdx.x;
+  // LDS slots of a thread's 16 points in the fixed mappings, as one base and
+  // a constant stride, so that they are addressed with immediate offsets
+  // instead of 16 loop-invariant address registers per mapping:
+  // pidx(e0 + i 2^S) = pidx(e0) + i (2^S + 2^(S-4)) when S >= 4 and bits
+  // S..S+3 of e0 are zero (level 0: S = h - 4; level 1: S = h - 8, so h = 12
+  // only; wl_row: S = 6; wl_run: consecutive slots).
+  uint64_t *const s0b = lds + pidx(celem<LH, 0, 4>(t, 0, 0));
+  uint64_t *const s1b = lds + pidx(celem<LH, 1, 4>(t, 0, 0));
+  uint64_t *const srunb = lds + pidx(wl_run(t, 0));
+  uint64_t *const srowb = lds + pidx(wl_row(t, 0));
+  auto s0 = [&](int i) -> uint64_t & { return s0b[i * ((1 << (LH - 4)) + (1 << (LH - 8)))]; };
+  auto s1 = [&](int i) -> uint64_t & {
+    if constexpr (LH >= 12) return s1b[i * ((1 << (LH - 8)) + (1 << (LH - 12)))];
+    else return lds[pidx(celem<LH, 1, 4>(t, 0, i))];
+  };
+  auto srun = [&](int k) -> uint64_t & { return srunb[k]; };
+  auto srow = [&](int i) -> uint64_t & { return srowb[i * 68]; };
 
   // level-0 twists: 15 per thread, read from the (cache-resident) level table
   // at each use, which keeps 30 VGPRs free
@@ -266,35 +283,35 @@ __device__ __forceinline__ void contig_run(const FastArgs &a, PassLds &sh, uint3
 #pragma unroll
       for (int i = 1; i < 16; ++i) x[i] = gl::mul(x[i], tw0[(i - 1) << (LH - 4)]);
 #pragma unroll
-      for (int i = 0; i < 16; ++i) lds[pidx(celem<LH, 0, 4>(t, 0, i))] = x[i];
+      for (int i = 0; i < 16; ++i) s0(i) = x[i];
       __syncthreads();
 #pragma unroll
-      for (int i = 0; i < 16; ++i) x[i] = lds[pidx(celem<LH, 1, 4>(t, 0, i))];
+      for (int i = 0; i < 16; ++i) x[i] = s1(i);
       dft_dif<4, INV>(x);
       if constexpr (WL) {
         level_twist_lds<LH - 4>(x, tw1, rp1);
         // each thread rewrites the slots it read, inside its wave's region
 #pragma unroll
-        for (int i = 0; i < 16; ++i) lds[pidx(celem<LH, 1, 4>(t, 0, i))] = x[i];
+        for (int i = 0; i < 16; ++i) s1(i) = x[i];
         wave_lds_sync();
 #pragma unroll
-        for (int k = 0; k < 16; ++k) x[k] = lds[pidx(wl_run(t, k))];
+        for (int k = 0; k < 16; ++k) x[k] = srun(k);
 #pragma unroll
         for (int g = 0; g < G2; ++g) dft_dif<A2, INV>(x + g * (1 << A2));
 #pragma unroll
-        for (int k = 0; k < 16; ++k) lds[pidx(wl_run(t, k))] = x[k];
+        for (int k = 0; k < 16; ++k) srun(k) = x[k];
         __syncthreads();  // the level-0 mapping crosses waves
 #pragma unroll
         for (int i = 0; i < 16; ++i) {
           const uint32_t e = celem<LH, 0, 4>(t, 0, i);
-          uint64_t v = lds[pidx(e)];
+          uint64_t v = s0(i);
           if constexpr (MODE == MODE_NINV) v = gl::mul(v, a.n_inv);
           __builtin_nontemporal_store(v, ct.dst(a, e));
         }
       } else if constexpr (NL == 3) {
         level_twist_lds<LH - 4>(x, tw1, rp1);
 #pragma unroll
-        for (int i = 0; i < 16; ++i) lds[pidx(celem<LH, 1, 4>(t, 0, i))] = x[i];
+        for (int i = 0; i < 16; ++i) s1(i) = x[i];
         __syncthreads();
 #pragma unroll
         for (int g = 0; g < G2; ++g)
@@ -315,7 +332,7 @@ __device__ __forceinline__ void contig_run(const FastArgs &a, PassLds &sh, uint3
 #pragma unroll
         for (int i = 0; i < 16; ++i) {
           const uint32_t e = celem<LH, 0, 4>(t, 0, i);
-          uint64_t v = lds[pidx(e)];
+          uint64_t v = s0(i);
           if constexpr (MODE == MODE_NINV) v = gl::mul(v, a.n_inv);
           if constexpr (MODE == MODE_LDE_OUT) v = gl::mul(v, gl::mul(cfb, sh.cf[i]));
           uint64_t *q = ct.dst(a, e);
@@ -325,25 +342,25 @@ __device__ __forceinline__ void contig_run(const FastArgs &a, PassLds &sh, uint3
     } else if constexpr (WL) {
       // loaded in the wl_row mapping: regroup into runs inside the wave's region
 #pragma unroll
-      for (int i = 0; i < 16; ++i) lds[pidx(wl_row(t, i))] = x[i];
+      for (int i = 0; i < 16; ++i) srow(i) = x[i];
       wave_lds_sync();
 #pragma unroll
-      for (int k = 0; k < 16; ++k) x[k] = lds[pidx(wl_run(t, k))];
+      for (int k = 0; k < 16; ++k) x[k] = srun(k);
 #pragma unroll
       for (int g = 0; g < G2; ++g) dft_dit<A2, INV>(x + g * (1 << A2));
 #pragma unroll
-      for (int k = 0; k < 16; ++k) lds[pidx(wl_run(t, k))] = x[k];
+      for (int k = 0; k < 16; ++k) srun(k) = x[k];
       wave_lds_sync();
 #pragma unroll
-      for (int i = 0; i < 16; ++i) x[i] = lds[pidx(celem<LH, 1, 4>(t, 0, i))];
+      for (int i = 0; i < 16; ++i) x[i] = s1(i);
       level_twist_lds<LH - 4>(x, tw1, rp1);
       dft_dit<4, INV>(x);
       // each thread rewrites the slots it read; the level-0 mapping crosses waves
 #pragma unroll
-      for (int i = 0; i < 16; ++i) lds[pidx(celem<LH, 1, 4>(t, 0, i))] = x[i];
+      for (int i = 0; i < 16; ++i) s1(i) = x[i];
       __syncthreads();
 #pragma unroll
-      for (int i = 0; i < 16; ++i) x[i] = lds[pidx(celem<LH, 0, 4>(t, 0, i))];
+      for (int i = 0; i < 16; ++i) x[i] = s0(i);
 #pragma unroll
       for (int i = 1; i < 16; ++i) x[i] = gl::mul(x[i], tw0[(i - 1) << (LH - 4)]);
       dft_dit<4, INV>(x);
@@ -357,7 +374,7 @@ __device__ __forceinline__ void contig_run(const FastArgs &a, PassLds &sh, uint3
     } else {
       // regroup the coalesced load into the last level's runs through LDS
 #pragma unroll
-      for (int i = 0; i < 16; ++i) lds[pidx(celem<LH, 0, 4>(t, 0, i))] = x[i];
+      for (int i = 0; i < 16; ++i) s0(i) = x[i];
       __syncthreads();
 #pragma unroll
       for (int g = 0; g < GL; ++g)
@@ -372,15 +389,15 @@ __device__ __forceinline__ void contig_run(const FastArgs &a, PassLds &sh, uint3
           for (int i = 0; i < (1 << A2); ++i) lds[pidx(celem<LH, 2, A2>(t, g, i))] = x[g * (1 << A2) + i];
         __syncthreads();
 #pragma unroll
-        for (int i = 0; i < 16; ++i) x[i] = lds[pidx(celem<LH, 1, 4>(t, 0, i))];
+        for (int i = 0; i < 16; ++i) x[i] = s1(i);
         level_twist_lds<LH - 4>(x, tw1, rp1);
       }
       dft_dit<4, INV>(x);
 #pragma unroll
-      for (int i = 0; i < 16; ++i) lds[pidx(celem<LH, 1, 4>(t, 0, i))] = x[i];
+      for (int i = 0; i < 16; ++i) s1(i) = x[i];
       __syncthreads();
 #pragma unroll
-      for (int i = 0; i < 16; ++i) x[i] = lds[pidx(celem<LH, 0, 4>(t, 0, i))];
+      for (int i = 0; i < 16; ++i) x[i] = s0(i);
 #pragma unroll
       for (int i = 1; i < 16; ++i) x[i] = gl::mul(x[i], tw0[(i - 1) << (LH - 4)]);
       dft_dit<4, INV>(x);
