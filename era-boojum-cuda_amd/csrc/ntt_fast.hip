@@ -9,12 +9,18 @@ namespace {
 using namespace pass;
 
 // Register bounds of three workgroups per CU for the DIF (n2b) variants and
-// two for the DIT (b2n) ones.  With their LDS slots addressed as one base
-// plus immediate offsets, the 12-bit kernels need 124 (n2b) and 142 (b2n)
-// VGPRs without spills and so run 4 and 3 workgroups per CU (LDS allows 4);
-// a bound of four workgroups would spill 14-26 dwords in the b2n variants.
+// two for the DIT (b2n) ones, except four for the plain and n^-1 12-bit b2n
+// kernels.  With their LDS slots addressed as one base plus immediate
+// offsets, the 12-bit n2b kernels need 124 VGPRs without spills and run 4
+// workgroups per CU (LDS allows 4).  The 12-bit b2n kernels need 142 (3 per
+// CU); held to 128 they spill 14 dwords and still ran 1.5-2% faster at 4 per
+// CU on 2^16-2^21 columns.  Their coset variants spill 20-26 dwords at 128
+// and were not faster, so they keep the bound of two.
 template <int LH, bool INV, bool DIT, int MODE, bool MC>
-__global__ __launch_bounds__(NT, DIT ? 2 : 3) void ntt_contig_kernel(FastArgs a) {
+__global__ __launch_bounds__(NT, DIT && LH == 12 && !MC && (MODE == MODE_NONE || MODE == MODE_NINV) ? 4
+                                 : DIT                                                           ? 2
+                                                                                                 : 3) void
+    ntt_contig_kernel(FastArgs a) {
   __shared__ PassLds sh;
   contig_run<LH, INV, DIT, MODE, MC>(a, sh, blockIdx.x, a.tiles, gridDim.x);
 }
